@@ -1,0 +1,114 @@
+/* sgm_hip.h -- the C ABI of libsgm_hip.so: Semi-Global Matching on one AMD
+ * Instinct MI355X (gfx950) GPU.
+ *
+ * Plain pointers and sizes only; no HIP or torch types cross this boundary
+ * (streams are passed as void*, a hipStream_t).  Every entry point returns 0 on
+ * success and -1 on failure; sgmh_last_error() then names the failing call with
+ * file:line, the way the reference's CK() macro reports CUDA errors
+ * (cuda/sgm_cuda_opt.cu:67-69).
+ *
+ * The reference exposes exactly one drop-in interface for this path: the
+ * harness ABI of common/sgm.h:27-40
+ *
+ *     int run(const uint8_t *left, const uint8_t *right, int W, int H,
+ *             uint8_t *disp, int threads, sgm_stage_times *t);
+ *
+ * with D and the path count fixed at compile time (common/sgm_params.h:13-18),
+ * plus the standalone multiscale binaries (multiscale/sgm_ms_ref.c:151-224).
+ * This library is what such an implementation calls; the one-line SGM_IMPL
+ * adapter is sgm-bench_amd/host/sgm_hip_impl.c.  D and the path count are
+ * runtime arguments here.
+ *
+ * Semantics (bit-exact to the reference oracles in every call):
+ *   plain       common/sgm_params.h:44-59 and common/sgm_ref.c:101-144:
+ *               9x7 census, Hamming cost (x-d<0 costs 62), P1=20, P2=192,
+ *               paths 4 (L->R, R->L, T->B, B->T) or 8 (+ diagonals), uint16
+ *               path sums, argmin with the lowest d winning ties.
+ *   multiscale  multiscale/sgm_ms_ref.c:47-202: two scales, Sobel-X prefilter,
+ *               invalid 63, cost merge with the half-resolution cost at
+ *               (x/2, y/2, d/2), 8 direction-weighted saturating paths
+ *               (P1 40/20/10, P2 200), output (W/2)x(H/2) decimated argmin.
+ *
+ * Constraints: 16 <= W, H <= 16384; D in {16, 32, ..., 256} (a multiple of 16,
+ * as common/sgm_params.h:37-39 requires); multiscale also needs even W and H
+ * (the reference's merge assumes it, sgm_ms_ref.c:184-188).
+ */
+#ifndef SGM_HIP_H
+#define SGM_HIP_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sgmh_ctx sgmh_ctx;
+
+/* Per-phase times in ms of the most recent call, measured with HIP events on
+ * the stream the kernels ran on (-1 = phase not run).  Mirrors the reference's
+ * sgm_stage_times (common/sgm.h:9-25); cost is fused into aggregation
+ * (cost_ms = -1) on the plain path, as in cuda/sgm_cuda_opt.cu:362. */
+typedef struct {
+    double census_ms;
+    double cost_ms;
+    double aggregate_ms;
+    double argmin_ms;
+    double transfer_ms;
+    double total_ms;
+} sgmh_times;
+
+/* Create / destroy a context bound to one device.  A context owns a stream and
+ * device workspaces that grow to the largest frame seen and are reused (the
+ * reference caches its device state in statics, cuda/sgm_cuda_opt.cu:292-319).
+ * A context is not thread-safe; calls on one context must be sequential. */
+int sgmh_ctx_create(sgmh_ctx **out, int device);
+void sgmh_ctx_destroy(sgmh_ctx *ctx);
+const char *sgmh_last_error(void);
+
+/* Per-phase HIP-event timing on (1) or off (0, default). */
+int sgmh_set_timing(sgmh_ctx *ctx, int on);
+/* Wait for the last call and read its phase times. */
+int sgmh_get_times(sgmh_ctx *ctx, sgmh_times *t);
+
+/* ---- plain SGM: replaces the body of sgm_run_fn (common/sgm.h:29-31) ---- */
+
+/* Host buffers, synchronous: disp is complete on return (common/sgm.h:27-31,
+ * the harness times the whole call including H2D/D2H, harness.c:154-158). */
+int sgmh_plain(sgmh_ctx *ctx, const uint8_t *left, const uint8_t *right,
+               int W, int H, int D, int paths, uint8_t *disp, sgmh_times *t);
+
+/* Device buffers, asynchronous on `stream` (NULL: the context's stream). */
+int sgmh_plain_dev(sgmh_ctx *ctx, const uint8_t *d_left, const uint8_t *d_right,
+                   int W, int H, int D, int paths, uint8_t *d_disp, void *stream);
+
+/* ---- multiscale (Configurations B/C): replaces sgm_ms_ref.c:166-202 ---- */
+
+/* disp is (W/2)*(H/2).  disp1 (may be NULL) receives the stage-1 half-resolution
+ * argmin that the reference computes and discards (sgm_ms_ref.c:175-180). */
+int sgmh_ms(sgmh_ctx *ctx, const uint8_t *left, const uint8_t *right,
+            int W, int H, int D, uint8_t *disp, uint8_t *disp1, sgmh_times *t);
+int sgmh_ms_dev(sgmh_ctx *ctx, const uint8_t *d_left, const uint8_t *d_right,
+                int W, int H, int D, uint8_t *d_disp, uint8_t *d_disp1, void *stream);
+
+/* ---- single phases, device buffers (for descriptor- and volume-level
+ *      parity against the oracle, the reference's common/test_phase1.c
+ *      pattern) ---- */
+
+/* 9x7 census of one image into W*H uint64 codes (sgm_ref.c:23-41). */
+int sgmh_census_dev(sgmh_ctx *ctx, const uint8_t *d_img, int W, int H,
+                    uint64_t *d_codes, void *stream);
+/* Sobel-X prefilter (sgm_ms_ref.c:47-54) and 2x2 half (sgm_ms_ref.c:145-150). */
+int sgmh_sobel_dev(sgmh_ctx *ctx, const uint8_t *d_img, int W, int H,
+                   uint8_t *d_out, void *stream);
+int sgmh_half_dev(sgmh_ctx *ctx, const uint8_t *d_img, int W, int H,
+                  uint8_t *d_out, void *stream);
+/* The uint8 path-cost volume L[(y*W+x)*D+d] of ONE direction (dx, dy), from
+ * census codes, plain-path rules (P1=20, P2=192, invalid 62, no saturation);
+ * the oracle's orc_path_L over orc_cost (sgm_ref.c:44-94). */
+int sgmh_path_plain_dev(sgmh_ctx *ctx, const uint64_t *d_cl, const uint64_t *d_cr,
+                        int W, int H, int D, int dx, int dy, uint8_t *d_L, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

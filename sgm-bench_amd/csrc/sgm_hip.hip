@@ -1,0 +1,1053 @@
+// sgm_hip.hip -- SGM kernels for gfx950 and the C ABI declared in
+// include/sgm_hip.h.
+//
+// Pipeline (plain path, common/sgm_ref.c:101-144):
+//   k_census        9x7 census of both images from an LDS-staged tile
+//   k_path<DPL,..>  every direction of the frame in ONE launch: one scanline
+//                   per 16-lane DPP row, the Hamming cost computed on the fly
+//                   from the census codes, the path cost L of that direction
+//                   written as uint8 to its own volume
+//   k_wta<DPL>      S = sum of the direction volumes (uint16 semantics) and
+//                   argmin with the lowest d winning ties, fused
+// Multiscale (multiscale/sgm_ms_ref.c:166-202) adds k_half, k_sobel and
+// k_cost (materialised cost volume, merged across scales) and runs k_path in
+// its cost-volume mode with saturation and direction-weighted P1.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+
+#include "sgm_hip.h"
+#include "sgm_kernels.h"
+
+using namespace sgmk;
+
+// ---------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+
+static void set_err(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+#define CK(call)                                                                     \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            set_err("HIP %s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+            return -1;                                                               \
+        }                                                                            \
+    } while (0)
+
+#define REQUIRE(cond, ...)                                                           \
+    do {                                                                             \
+        if (!(cond)) {                                                               \
+            set_err(__VA_ARGS__);                                                    \
+            return -1;                                                               \
+        }                                                                            \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// parameters fixed by the reference
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPlainP1 = 20;      // common/sgm_params.h:23
+constexpr uint32_t kPlainP2 = 192;     // common/sgm_params.h:24
+constexpr uint32_t kPlainInvalid = 62; // common/sgm_params.h:27-29
+constexpr uint32_t kMsP2 = 200;        // multiscale/sgm_ms_ref.c:39
+constexpr uint32_t kMsInvalid = 63;    // multiscale/sgm_ms_ref.c:41
+
+// common/sgm_ref.c:96-99
+static const int kDirsPlain[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1},
+                                     {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+// multiscale/sgm_ms_ref.c:85-87: dx, dy, P1
+static const int kDirsMs[8][3] = {{1, 0, 40}, {-1, 0, 40}, {0, 1, 20}, {0, -1, 20},
+                                  {1, 1, 10}, {-1, -1, 10}, {1, -1, 10}, {-1, 1, 10}};
+
+// ---------------------------------------------------------------------------
+// census: 64x16 output tile per 256-thread block, each thread 4 consecutive x
+// ---------------------------------------------------------------------------
+constexpr int kCenTX = 64, kCenTY = 16;
+constexpr int kCenRowBytes = kCenTX + 8;         // 72: x0-4 .. x0+67
+constexpr int kCenRows = kCenTY + 6;             // 22: y0-3 .. y0+18
+constexpr int kCenRowWords = kCenRowBytes / 4;   // 18
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__global__ __launch_bounds__(256) void k_census(const uint8_t *__restrict__ img,
+                                                uint64_t *__restrict__ out, int W, int H,
+                                                int ostride)
+{
+    __shared__ uint32_t tile[kCenRows * kCenRowWords];
+    uint8_t *t8 = reinterpret_cast<uint8_t *>(tile);
+    const int x0 = blockIdx.x * kCenTX, y0 = blockIdx.y * kCenTY;
+    for (int i = threadIdx.x; i < kCenRows * kCenRowBytes; i += 256) {
+        const int r = i / kCenRowBytes, c = i - r * kCenRowBytes;
+        const int yy = clampi(y0 - 3 + r, 0, H - 1), xx = clampi(x0 - 4 + c, 0, W - 1);
+        t8[i] = img[(size_t)yy * W + xx];
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int x = x0 + 4 * tx, y = y0 + ty;
+    if (y >= H || x >= W)
+        return;
+    // bytes 4tx .. 4tx+11 of the 7 tile rows around this thread's row
+    uint32_t win[7][3];
+#pragma unroll
+    for (int r = 0; r < 7; r++)
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            win[r][k] = tile[(ty + r) * kCenRowWords + tx + k];
+    uint64_t code[4];
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const uint32_t centre = (win[3][(p + 4) >> 2] >> (8 * ((p + 4) & 3))) & 0xFFu;
+        uint32_t lo = 0, hi = 0;
+        int n = 0;
+        // bit n: n-th neighbour in (dy outer -3..3, dx inner -4..4) order,
+        // centre skipped (common/sgm_params.h:45-47)
+#pragma unroll
+        for (int r = 0; r < 7; r++)
+#pragma unroll
+            for (int c = 0; c < 9; c++) {
+                if (r == 3 && c == 4)
+                    continue;
+                const int b = p + c; // byte index of (x+p + c-4) in the window
+                const uint32_t nb = (win[r][b >> 2] >> (8 * (b & 3))) & 0xFFu;
+                const uint32_t bit = nb < centre ? 1u : 0u;
+                if (n < 32)
+                    lo |= bit << n;
+                else
+                    hi |= bit << (n - 32);
+                n++;
+            }
+        code[p] = ((uint64_t)hi << 32) | lo;
+    }
+    uint64_t *o = out + (size_t)y * ostride + x;
+    if (x + 3 < W) {
+        o[0] = code[0];
+        o[1] = code[1];
+        o[2] = code[2];
+        o[3] = code[3];
+    } else {
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+            if (x + p < W)
+                o[p] = code[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// multiscale pre-filters
+// ---------------------------------------------------------------------------
+// 2x2 box mean with +2 rounding, multiscale/sgm_ms_ref.c:145-150
+__global__ __launch_bounds__(256) void k_half(const uint8_t *__restrict__ img, int W, int H,
+                                              uint8_t *__restrict__ out)
+{
+    const int w = W / 2, h = H / 2;
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= w || y >= h)
+        return;
+    const uint8_t *a = img + (size_t)(2 * y) * W + 2 * x;
+    out[(size_t)y * w + x] = (uint8_t)((a[0] + a[1] + a[W] + a[W + 1] + 2) >> 2);
+}
+
+// Sobel-X, out = clamp(128 + (gx >> 3)), multiscale/sgm_ms_ref.c:47-54
+__global__ __launch_bounds__(256) void k_sobel(const uint8_t *__restrict__ img, int W, int H,
+                                               uint8_t *__restrict__ out)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H)
+        return;
+    const int xl = clampi(x - 1, 0, W - 1), xr = clampi(x + 1, 0, W - 1);
+    const uint8_t *up = img + (size_t)clampi(y - 1, 0, H - 1) * W;
+    const uint8_t *md = img + (size_t)y * W;
+    const uint8_t *dn = img + (size_t)clampi(y + 1, 0, H - 1) * W;
+    const int gx = (up[xr] - up[xl]) + 2 * (md[xr] - md[xl]) + (dn[xr] - dn[xl]);
+    out[(size_t)y * W + x] = (uint8_t)clampi(128 + (gx >> 3), 0, 255);
+}
+
+// Materialised cost volume (multiscale): one pixel per 16-lane row.
+//   merge == nullptr: C = popcount(cl ^ cr[x-d]) or `invalid`      (sgm_ms_ref.c:66-72)
+//   merge != nullptr: C = (C + merge[(y/2)*hw + x/2][d/2] + 1) >> 1 (sgm_ms_ref.c:184-188)
+template <int DPL>
+__global__ __launch_bounds__(256) void k_cost(const uint64_t *__restrict__ cl,
+                                              const uint64_t *__restrict__ cr, int W, int H, int D,
+                                              uint32_t invalid, const uint8_t *__restrict__ merge,
+                                              uint8_t *__restrict__ C)
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+    const size_t p = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + g;
+    const size_t N = (size_t)W * H;
+    if (p >= N)
+        return;
+    const int y = (int)(p / W), x = (int)(p - (size_t)y * W);
+    const uint64_t a = cl[p];
+    const uint64_t *crow = cr + (size_t)y * W;
+    uint32_t c[DPL];
+#pragma unroll
+    for (int j = 0; j < DPL; j++) {
+        const int xr = x - (l * DPL + j);
+        const uint64_t b = crow[xr < 0 ? 0 : xr];
+        c[j] = xr < 0 ? invalid : (uint32_t)__popcll(a ^ b);
+    }
+    if (merge) {
+        const int hw = W / 2;
+        const uint8_t *m = merge + ((size_t)(y / 2) * hw + x / 2) * D;
+#pragma unroll
+        for (int j = 0; j < DPL; j++)
+            c[j] = (c[j] + m[(l * DPL + j) >> 1] + 1) >> 1;
+    }
+    store_bytes<DPL>(C + p * D + l * DPL, c);
+}
+
+// ---------------------------------------------------------------------------
+// path aggregation: all directions of a frame in one launch
+// ---------------------------------------------------------------------------
+// Census codes handed to the aggregation kernels are row-padded: row y starts
+// at codes + y*cstride and kCensusPad codes of slack precede every row, so a
+// lane may read cr[y][x-d] for any d < 256 without a bounds branch (the value
+// read for x-d < 0 is discarded by the invalid-cost select).
+constexpr int kCensusPad = 256;
+
+struct DirTask {
+    int dx, dy, P1;
+    int nlines;      // scanlines in this direction
+    int first_block; // first block of this direction in the grid
+    uint8_t *L;      // output volume of this direction
+};
+
+struct PathArgs {
+    const uint64_t *cl, *cr; // padded census codes (cost computed on the fly)
+    const uint8_t *cost;     // or a materialised cost volume
+    int W, H, D, cstride;
+    uint32_t P2, invalid;
+    int ndirs;
+    DirTask dir[8];
+};
+
+struct LineGeom {
+    int x, y, steps;
+};
+
+// Start pixel and length of scanline i of direction (dx, dy).  Lines of an
+// axis-aligned direction are the rows / columns; a diagonal direction has
+// W+H-1 lines starting on the edge row its predecessors fall off, then on the
+// edge column (multiscale/sgm_ms_cuda.cu:91-98 enumerates the same set).
+__device__ __forceinline__ LineGeom line_geom(int i, int dx, int dy, int W, int H)
+{
+    if (dy == 0)
+        return {dx > 0 ? 0 : W - 1, i, W};
+    if (dx == 0)
+        return {i, dy > 0 ? 0 : H - 1, H};
+    int x, y;
+    if (i < W) {
+        x = dx > 0 ? i : W - 1 - i;
+        y = dy > 0 ? 0 : H - 1;
+    } else {
+        const int j = i - W + 1;
+        x = dx > 0 ? 0 : W - 1;
+        y = dy > 0 ? j : H - 1 - j;
+    }
+    const int sx = dx > 0 ? W - x : x + 1, sy = dy > 0 ? H - y : y + 1;
+    return {x, y, sx < sy ? sx : sy};
+}
+
+__device__ __forceinline__ uint64_t dpp64_shr1(uint64_t old, uint64_t v)
+{
+    const uint32_t lo = dpp_or<kRowShr1>((uint32_t)old, (uint32_t)v);
+    const uint32_t hi = dpp_or<kRowShr1>((uint32_t)(old >> 32), (uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t dpp64_shl1(uint64_t old, uint64_t v)
+{
+    const uint32_t lo = dpp_or<kRowShl1>((uint32_t)old, (uint32_t)v);
+    const uint32_t hi = dpp_or<kRowShl1>((uint32_t)(old >> 32), (uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <int DPL, bool SAT>
+__device__ __forceinline__ void advance(uint32_t (&L)[DPL], const uint32_t (&C)[DPL], uint32_t &mn,
+                                        bool first, uint32_t P1, uint32_t P2)
+{
+    if (first) {
+#pragma unroll
+        for (int j = 0; j < DPL; j++)
+            L[j] = C[j];
+        mn = row_min16(lane_min<DPL>(L));
+    } else {
+        sgm_step<DPL, SAT>(L, C, mn, P1, P2);
+    }
+}
+
+// Horizontal direction, cost from census codes.  Lane l keeps the window
+// w[j] = cr[y][x - l*DPL - j] in registers; one step along the row shifts the
+// window by one disparity: the code leaving lane l's slice enters lane l+1's
+// (L->R) or lane l-1's (R->L) through one row_shr:1 / row_shl:1 DPP move, and
+// the end lane takes the one new code of the step.  The window lives in a
+// register ring indexed by the step modulo DPL, so the shift costs no moves.
+template <int DPL, bool SAT>
+__device__ __forceinline__ void path_horizontal(const PathArgs &a, const DirTask &t, LineGeom ln,
+                                                int l, int maxsteps)
+{
+    constexpr int PF = DPL > 8 ? DPL : 8; // prefetch distance in steps; a multiple of DPL
+    const int W = a.W, D = a.D, dx = t.dx;
+    const uint32_t P1 = (uint32_t)t.P1, P2 = a.P2, inval = a.invalid;
+    const bool live = ln.steps > 0;
+    const int y = live ? ln.y : 0;
+    const uint64_t *clrow = a.cl + (size_t)y * a.cstride;
+    const uint64_t *crrow = a.cr + (size_t)y * a.cstride;
+    uint8_t *out = t.L + ((size_t)y * W + ln.x) * D + l * DPL;
+    const long ostep = (long)dx * D;
+    const int lbase = l * DPL;
+
+    // phys[(j + s) mod DPL] holds logical w[j] at step s for R->L; for L->R the
+    // logical index runs the other way: phys[(j - s) mod DPL].  PF is a
+    // multiple of DPL, so s mod DPL == k mod DPL inside the unrolled block.
+    uint64_t phys[DPL];
+    int x = ln.x;
+#pragma unroll
+    for (int j = 0; j < DPL; j++)
+        phys[j] = crrow[x - lbase - j];
+    uint64_t pl[PF], pn[PF]; // prefetched cl[x_s] and the new window code of step s
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        const int xs = x + k * dx;
+        const bool ok = live && k < ln.steps;
+        pl[k] = ok ? clrow[xs] : 0;
+        pn[k] = ok ? crrow[dx > 0 ? xs : xs - (D - 1)] : 0;
+    }
+    uint32_t L[DPL], mn = 0;
+    for (int s0 = 0; s0 < maxsteps; s0 += PF) {
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int s = s0 + k;
+            if (s < ln.steps) {
+                if (s > 0) {
+                    // shift the window by one step
+                    if (dx > 0) {
+                        const int slot = ((-k) % DPL + DPL) % DPL; // logical 0 at step s
+                        phys[slot] = dpp64_shr1(pn[k], phys[slot]);    // was logical DPL-1
+                    } else {
+                        const int slot = ((k - 1) % DPL + DPL) % DPL; // logical DPL-1 now
+                        phys[slot] = dpp64_shl1(pn[k], phys[slot]);       // was logical 0
+                    }
+                }
+                const uint64_t cL = pl[k];
+                const int xs = x;
+                // prefetch step s + PF
+                {
+                    const int sp = s + PF;
+                    const bool ok = sp < ln.steps;
+                    const int xp = xs + PF * dx;
+                    pl[k] = ok ? clrow[xp] : 0;
+                    pn[k] = ok ? crrow[dx > 0 ? xp : xp - (D - 1)] : 0;
+                }
+                uint32_t C[DPL];
+#pragma unroll
+                for (int j = 0; j < DPL; j++) {
+                    const int slot = dx > 0 ? (((j - k) % DPL) + DPL) % DPL : (j + k) % DPL;
+                    const uint32_t h = (uint32_t)__popcll(cL ^ phys[slot]);
+                    C[j] = xs - lbase - j >= 0 ? h : inval;
+                }
+                advance<DPL, SAT>(L, C, mn, s == 0, P1, P2);
+                store_bytes<DPL>(out, L);
+                out += ostep;
+                x += dx;
+            }
+        }
+    }
+}
+
+// Any direction: cost from census codes read per lane, or from a materialised
+// cost volume, with a prefetch ring PF steps deep.  DECIM stores L only at
+// even (x, y), into a (W/2)x(H/2) layout (the multiscale output needs S only
+// there, sgm_ms_ref.c:194-199).
+template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM>
+__device__ __forceinline__ void path_generic(const PathArgs &a, const DirTask &t, LineGeom ln, int l,
+                                             int maxsteps)
+{
+    constexpr int PF = DPL <= 4 ? 4 : 2;
+    constexpr int NV = FROM_CENSUS ? DPL + 1 : (DPL + 3) / 4; // 64- or 32-bit words per step
+    using word = typename std::conditional<FROM_CENSUS, uint64_t, uint32_t>::type;
+    const int W = a.W, D = a.D, dx = t.dx, dy = t.dy;
+    const uint32_t P1 = (uint32_t)t.P1, P2 = a.P2, inval = a.invalid;
+    const int lbase = l * DPL;
+    const int hw = W >> 1;
+    int x = ln.x, y = ln.y;
+
+    auto fetch = [&](int xs, int ys, word (&v)[NV]) {
+        if constexpr (FROM_CENSUS) {
+            const uint64_t *cr = a.cr + (size_t)ys * a.cstride + xs - lbase;
+            v[DPL] = a.cl[(size_t)ys * a.cstride + xs];
+#pragma unroll
+            for (int j = 0; j < DPL; j++)
+                v[j] = cr[-j];
+        } else {
+            const uint8_t *p = a.cost + ((size_t)ys * W + xs) * D + lbase;
+            if constexpr (DPL == 1) {
+                v[0] = *p;
+            } else if constexpr (DPL == 2) {
+                v[0] = *reinterpret_cast<const uint16_t *>(p);
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; q++)
+                    v[q] = reinterpret_cast<const uint32_t *>(p)[q];
+            }
+        }
+    };
+
+    word ring[PF][NV];
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        if (k < ln.steps)
+            fetch(x + k * dx, y + k * dy, ring[k]);
+    }
+    uint32_t L[DPL], mn = 0;
+    for (int s0 = 0; s0 < maxsteps; s0 += PF) {
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int s = s0 + k;
+            if (s < ln.steps) {
+                uint32_t C[DPL];
+                if constexpr (FROM_CENSUS) {
+                    const uint64_t cL = ring[k][DPL];
+#pragma unroll
+                    for (int j = 0; j < DPL; j++) {
+                        const uint32_t h = (uint32_t)__popcll(cL ^ ring[k][j]);
+                        C[j] = x - lbase - j >= 0 ? h : inval;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < DPL; j++)
+                        C[j] = (ring[k][j / 4] >> (8 * (j % 4))) & 0xFFu;
+                }
+                if (s + PF < ln.steps)
+                    fetch(x + PF * dx, y + PF * dy, ring[k]);
+                advance<DPL, SAT>(L, C, mn, s == 0, P1, P2);
+                if constexpr (!DECIM) {
+                    store_bytes<DPL>(t.L + ((size_t)y * W + x) * D + lbase, L);
+                } else {
+                    if (((x | y) & 1) == 0)
+                        store_bytes<DPL>(t.L + ((size_t)(y >> 1) * hw + (x >> 1)) * D + lbase, L);
+                }
+                x += dx;
+                y += dy;
+            }
+        }
+    }
+}
+
+template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM>
+__global__ __launch_bounds__(64) void k_path(PathArgs a)
+{
+    const int lane = threadIdx.x, g = lane >> 4, l = lane & 15;
+    int k = 0;
+    while (k + 1 < a.ndirs && (int)blockIdx.x >= a.dir[k + 1].first_block)
+        k++;
+    const DirTask t = a.dir[k];
+    const int base = ((int)blockIdx.x - t.first_block) * kLinesPerWave;
+    int maxsteps = 0;
+#pragma unroll
+    for (int q = 0; q < kLinesPerWave; q++)
+        if (base + q < t.nlines)
+            maxsteps = max(maxsteps, line_geom(base + q, t.dx, t.dy, a.W, a.H).steps);
+    const int line = base + g;
+    LineGeom ln = {0, 0, 0};
+    if (line < t.nlines)
+        ln = line_geom(line, t.dx, t.dy, a.W, a.H);
+    if constexpr (FROM_CENSUS && !DECIM) {
+        if (t.dy == 0) {
+            path_horizontal<DPL, SAT>(a, t, ln, l, maxsteps);
+            return;
+        }
+    }
+    path_generic<DPL, FROM_CENSUS, SAT, DECIM>(a, t, ln, l, maxsteps);
+}
+
+// ---------------------------------------------------------------------------
+// S = sum over directions, argmin (lowest d wins ties), sgm_ref.c:117-124
+// ---------------------------------------------------------------------------
+template <int DPL>
+__global__ __launch_bounds__(256) void k_wta(const uint8_t *__restrict__ L, size_t dir_stride,
+                                             int ndirs, size_t npix, int D, uint8_t *__restrict__ disp)
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+    const size_t stride = (size_t)gridDim.x * 16;
+    for (size_t p = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + g; p < npix; p += stride) {
+        uint32_t s[DPL];
+#pragma unroll
+        for (int j = 0; j < DPL; j++)
+            s[j] = 0;
+        const uint8_t *src = L + p * D + l * DPL;
+        for (int k = 0; k < ndirs; k++) {
+            uint32_t v[DPL];
+            load_bytes<DPL>(src + k * dir_stride, v);
+#pragma unroll
+            for (int j = 0; j < DPL; j++)
+                s[j] += v[j];
+        }
+        // key = S << 8 | d: the smallest key is the smallest S, then the smallest d
+        uint32_t key = 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < DPL; j++)
+            key = umin(key, (s[j] << 8) | (uint32_t)(l * DPL + j));
+        key = row_min16(key);
+        if (l == 0)
+            disp[p] = (uint8_t)(key & 0xFFu);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+enum Phase { PH_CENSUS = 0, PH_COST, PH_AGG, PH_ARGMIN, PH_XFER, PH_N };
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct sgmh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, std::pair<int, int>>> marks; // phase, ev begin, ev end
+    hipStream_t last_stream = nullptr;
+    // workspaces
+    DevBuf img_l, img_r, disp, disp1;          // host-buffer entry points
+    DevBuf cl, cr;                             // census codes
+    DevBuf vol;                                // direction volumes
+    DevBuf half_l, half_r, sob_l, sob_r;       // multiscale images
+    DevBuf cl1, cr1, cost1, cost2, vol1;       // multiscale stage buffers
+};
+
+static int ensure(DevBuf &b, size_t bytes)
+{
+    if (b.cap >= bytes)
+        return 0;
+    if (b.p) {
+        CK(hipDeviceSynchronize());
+        CK(hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    CK(hipMalloc(&b.p, bytes));
+    b.cap = bytes;
+    return 0;
+}
+
+static void free_buf(DevBuf &b)
+{
+    if (b.p)
+        (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// Phase timing: events are recorded only when timing is on.
+struct PhaseScope {
+    sgmh_ctx *ctx;
+    hipStream_t s;
+    int phase, ev0 = -1;
+    PhaseScope(sgmh_ctx *c, hipStream_t st, int ph) : ctx(c), s(st), phase(ph)
+    {
+        if (ctx->timing)
+            ev0 = record();
+    }
+    int record()
+    {
+        if (ctx->ev_used == ctx->events.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess)
+                return -1;
+            ctx->events.push_back(e);
+        }
+        const int i = (int)ctx->ev_used++;
+        if (hipEventRecord(ctx->events[i], s) != hipSuccess)
+            return -1;
+        return i;
+    }
+    ~PhaseScope()
+    {
+        if (ctx->timing && ev0 >= 0) {
+            const int ev1 = record();
+            if (ev1 >= 0)
+                ctx->marks.push_back({phase, {ev0, ev1}});
+        }
+    }
+};
+
+static void begin_call(sgmh_ctx *ctx, hipStream_t s)
+{
+    ctx->ev_used = 0;
+    ctx->marks.clear();
+    ctx->last_stream = s;
+}
+
+static int check_dims(int W, int H, int D)
+{
+    REQUIRE(W >= 16 && H >= 16 && W <= 16384 && H <= 16384,
+            "sgmh: image %dx%d outside 16..16384", W, H);
+    REQUIRE(D == 16 || D == 32 || D == 64 || D == 128 || D == 256,
+            "sgmh: D=%d unsupported (16, 32, 64, 128 or 256)", D);
+    return 0;
+}
+
+static int launch_census(hipStream_t s, const uint8_t *img, uint64_t *out, int W, int H,
+                         int ostride)
+{
+    dim3 grid((W + kCenTX - 1) / kCenTX, (H + kCenTY - 1) / kCenTY);
+    hipLaunchKernelGGL(k_census, grid, dim3(256), 0, s, img, out, W, H, ostride);
+    CK(hipGetLastError());
+    return 0;
+}
+
+template <bool FROM_CENSUS, bool SAT, bool DECIM>
+static int launch_path_t(hipStream_t s, const PathArgs &a, int blocks)
+{
+    switch (a.D / 16) {
+    case 1: hipLaunchKernelGGL((k_path<1, FROM_CENSUS, SAT, DECIM>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_path<2, FROM_CENSUS, SAT, DECIM>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_path<4, FROM_CENSUS, SAT, DECIM>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_path<8, FROM_CENSUS, SAT, DECIM>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_path<16, FROM_CENSUS, SAT, DECIM>), dim3(blocks), dim3(64), 0, s, a); break;
+    default: set_err("sgmh: bad D %d", a.D); return -1;
+    }
+    CK(hipGetLastError());
+    return 0;
+}
+
+// Fill the direction table (longest lines first, so the first blocks the
+// dispatcher places are the longest-running ones) and launch.  Two variants
+// are instantiated: census-fed plain paths and cost-volume-fed saturating
+// multiscale paths (full or decimated store).
+static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndirs,
+                        bool from_census, bool sat, uint8_t *vol, size_t dir_stride, int decimate)
+{
+    int order[8];
+    for (int k = 0; k < ndirs; k++)
+        order[k] = k;
+    auto len = [&](int k) { return dirs[k][1] == 0 ? a.W : (dirs[k][0] == 0 ? a.H : std::min(a.W, a.H)); };
+    std::stable_sort(order, order + ndirs, [&](int p, int q) { return len(p) > len(q); });
+    int blocks = 0;
+    a.ndirs = ndirs;
+    for (int i = 0; i < ndirs; i++) {
+        const int k = order[i];
+        DirTask &t = a.dir[i];
+        t.dx = dirs[k][0];
+        t.dy = dirs[k][1];
+        t.P1 = dirs[k][2];
+        t.nlines = t.dy == 0 ? a.H : (t.dx == 0 ? a.W : a.W + a.H - 1);
+        t.first_block = blocks;
+        t.L = vol + (size_t)k * dir_stride;
+        blocks += (t.nlines + kLinesPerWave - 1) / kLinesPerWave;
+    }
+    if (from_census && !sat && !decimate)
+        return launch_path_t<true, false, false>(s, a, blocks);
+    if (!from_census && sat && !decimate)
+        return launch_path_t<false, true, false>(s, a, blocks);
+    if (!from_census && sat && decimate)
+        return launch_path_t<false, true, true>(s, a, blocks);
+    set_err("sgmh: unsupported path-kernel variant");
+    return -1;
+}
+
+static int launch_wta(hipStream_t s, const uint8_t *vol, size_t dir_stride, int ndirs, size_t npix,
+                      int D, uint8_t *disp)
+{
+    const size_t groups = (npix + 15) / 16;
+    const int blocks = (int)std::min<size_t>(groups, 65536);
+    switch (D / 16) {
+    case 1: hipLaunchKernelGGL(k_wta<1>, dim3(blocks), dim3(256), 0, s, vol, dir_stride, ndirs, npix, D, disp); break;
+    case 2: hipLaunchKernelGGL(k_wta<2>, dim3(blocks), dim3(256), 0, s, vol, dir_stride, ndirs, npix, D, disp); break;
+    case 4: hipLaunchKernelGGL(k_wta<4>, dim3(blocks), dim3(256), 0, s, vol, dir_stride, ndirs, npix, D, disp); break;
+    case 8: hipLaunchKernelGGL(k_wta<8>, dim3(blocks), dim3(256), 0, s, vol, dir_stride, ndirs, npix, D, disp); break;
+    case 16: hipLaunchKernelGGL(k_wta<16>, dim3(blocks), dim3(256), 0, s, vol, dir_stride, ndirs, npix, D, disp); break;
+    default: set_err("sgmh: bad D %d", D); return -1;
+    }
+    CK(hipGetLastError());
+    return 0;
+}
+
+static int launch_cost(hipStream_t s, const uint64_t *cl, const uint64_t *cr, int W, int H, int D,
+                       uint32_t invalid, const uint8_t *merge, uint8_t *C)
+{
+    const size_t N = (size_t)W * H;
+    const int blocks = (int)((N + 15) / 16);
+    switch (D / 16) {
+    case 1: hipLaunchKernelGGL(k_cost<1>, dim3(blocks), dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
+    case 2: hipLaunchKernelGGL(k_cost<2>, dim3(blocks), dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
+    case 4: hipLaunchKernelGGL(k_cost<4>, dim3(blocks), dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
+    case 8: hipLaunchKernelGGL(k_cost<8>, dim3(blocks), dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
+    case 16: hipLaunchKernelGGL(k_cost<16>, dim3(blocks), dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
+    default: set_err("sgmh: bad D %d", D); return -1;
+    }
+    CK(hipGetLastError());
+    return 0;
+}
+
+static hipStream_t pick(sgmh_ctx *ctx, void *stream)
+{
+    return stream ? (hipStream_t)stream : ctx->stream;
+}
+
+// Padded census layout for the aggregation kernels: kCensusPad codes before
+// row 0 and a row stride of W + kCensusPad.
+static size_t census_padded_codes(int W, int H) { return (size_t)kCensusPad + (size_t)H * (W + kCensusPad); }
+
+static int plain_paths(sgmh_ctx *ctx, hipStream_t s, const uint64_t *cl, const uint64_t *cr, int W,
+                       int H, int D, const int (*dirs)[3], int ndirs, uint8_t *vol, size_t dir_stride)
+{
+    (void)ctx;
+    PathArgs a{};
+    a.cl = cl;
+    a.cr = cr;
+    a.W = W;
+    a.H = H;
+    a.D = D;
+    a.cstride = W + kCensusPad;
+    a.P2 = kPlainP2;
+    a.invalid = kPlainInvalid;
+    return launch_paths(s, a, dirs, ndirs, true, false, vol, dir_stride, 0);
+}
+
+static int plain_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W,
+                         int H, int D, int paths, uint8_t *disp)
+{
+    const size_t N = (size_t)W * H, NC = census_padded_codes(W, H);
+    if (ensure(ctx->cl, NC * 8) || ensure(ctx->cr, NC * 8) || ensure(ctx->vol, N * D * paths))
+        return -1;
+    uint64_t *cl = (uint64_t *)ctx->cl.p + kCensusPad, *cr = (uint64_t *)ctx->cr.p + kCensusPad;
+    const int cstride = W + kCensusPad;
+    uint8_t *vol = (uint8_t *)ctx->vol.p;
+    {
+        PhaseScope ps(ctx, s, PH_CENSUS);
+        if (launch_census(s, l, cl, W, H, cstride) || launch_census(s, r, cr, W, H, cstride))
+            return -1;
+    }
+    {
+        PhaseScope ps(ctx, s, PH_AGG);
+        int dirs[8][3];
+        for (int k = 0; k < paths; k++) {
+            dirs[k][0] = kDirsPlain[k][0];
+            dirs[k][1] = kDirsPlain[k][1];
+            dirs[k][2] = (int)kPlainP1;
+        }
+        if (plain_paths(ctx, s, cl, cr, W, H, D, dirs, paths, vol, N * D))
+            return -1;
+    }
+    {
+        PhaseScope ps(ctx, s, PH_ARGMIN);
+        if (launch_wta(s, vol, N * D, paths, N, D, disp))
+            return -1;
+    }
+    return 0;
+}
+
+static int ms_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W, int H,
+                      int D, uint8_t *disp, uint8_t *disp1)
+{
+    const int hw = W / 2, hh = H / 2;
+    const size_t N = (size_t)W * H, Nh = (size_t)hw * hh;
+    if (ensure(ctx->half_l, Nh) || ensure(ctx->half_r, Nh) || ensure(ctx->sob_l, N) ||
+        ensure(ctx->sob_r, N) || ensure(ctx->cl1, Nh * 8) || ensure(ctx->cr1, Nh * 8) ||
+        ensure(ctx->cl, N * 8) || ensure(ctx->cr, N * 8) || ensure(ctx->cost1, Nh * D) ||
+        ensure(ctx->cost2, N * D) || ensure(ctx->vol1, Nh * D * 8) || ensure(ctx->vol, Nh * D * 8) ||
+        (disp1 == nullptr && ensure(ctx->disp1, Nh)))
+        return -1;
+    uint8_t *hl = (uint8_t *)ctx->half_l.p, *hr = (uint8_t *)ctx->half_r.p;
+    uint8_t *sl = (uint8_t *)ctx->sob_l.p, *sr = (uint8_t *)ctx->sob_r.p;
+    uint64_t *cl1 = (uint64_t *)ctx->cl1.p, *cr1 = (uint64_t *)ctx->cr1.p;
+    uint64_t *cl2 = (uint64_t *)ctx->cl.p, *cr2 = (uint64_t *)ctx->cr.p;
+    uint8_t *C1 = (uint8_t *)ctx->cost1.p, *C2 = (uint8_t *)ctx->cost2.p;
+    uint8_t *V1 = (uint8_t *)ctx->vol1.p, *V2 = (uint8_t *)ctx->vol.p;
+    uint8_t *d1 = disp1 ? disp1 : (uint8_t *)ctx->disp1.p;
+    int dirs[8][3];
+    memcpy(dirs, kDirsMs, sizeof dirs);
+    {
+        PhaseScope ps(ctx, s, PH_CENSUS);
+        const dim3 gh((hw + 63) / 64, (hh + 3) / 4), gf((W + 63) / 64, (H + 3) / 4);
+        hipLaunchKernelGGL(k_half, gh, dim3(256), 0, s, l, W, H, hl);
+        hipLaunchKernelGGL(k_half, gh, dim3(256), 0, s, r, W, H, hr);
+        hipLaunchKernelGGL(k_sobel, dim3((hw + 63) / 64, (hh + 3) / 4), dim3(256), 0, s, hl, hw, hh, sl);
+        hipLaunchKernelGGL(k_sobel, dim3((hw + 63) / 64, (hh + 3) / 4), dim3(256), 0, s, hr, hw, hh, sr);
+        CK(hipGetLastError());
+        if (launch_census(s, sl, cl1, hw, hh, hw) || launch_census(s, sr, cr1, hw, hh, hw))
+            return -1;
+        hipLaunchKernelGGL(k_sobel, gf, dim3(256), 0, s, l, W, H, sl);
+        hipLaunchKernelGGL(k_sobel, gf, dim3(256), 0, s, r, W, H, sr);
+        CK(hipGetLastError());
+        if (launch_census(s, sl, cl2, W, H, W) || launch_census(s, sr, cr2, W, H, W))
+            return -1;
+    }
+    {
+        PhaseScope ps(ctx, s, PH_COST);
+        if (launch_cost(s, cl1, cr1, hw, hh, D, kMsInvalid, nullptr, C1) ||
+            launch_cost(s, cl2, cr2, W, H, D, kMsInvalid, C1, C2))
+            return -1;
+    }
+    PathArgs a{};
+    a.D = D;
+    a.P2 = kMsP2;
+    a.invalid = kMsInvalid;
+    {
+        PhaseScope ps(ctx, s, PH_AGG);
+        a.cost = C1;
+        a.W = hw;
+        a.H = hh;
+        if (launch_paths(s, a, dirs, 8, false, true, V1, Nh * D, 0))
+            return -1;
+        a.cost = C2;
+        a.W = W;
+        a.H = H;
+        if (launch_paths(s, a, dirs, 8, false, true, V2, Nh * D, 1))
+            return -1;
+    }
+    {
+        PhaseScope ps(ctx, s, PH_ARGMIN);
+        if (launch_wta(s, V1, Nh * D, 8, Nh, D, d1) || launch_wta(s, V2, Nh * D, 8, Nh, D, disp))
+            return -1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *sgmh_last_error(void) { return g_err; }
+
+int sgmh_ctx_create(sgmh_ctx **out, int device)
+{
+    REQUIRE(out != nullptr, "sgmh_ctx_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    CK(hipGetDeviceCount(&n));
+    REQUIRE(device >= 0 && device < n, "sgmh_ctx_create: device %d of %d", device, n);
+    CK(hipSetDevice(device));
+    sgmh_ctx *c = new sgmh_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_err("sgmh_ctx_create: hipStreamCreateWithFlags failed");
+        delete c;
+        return -1;
+    }
+    *out = c;
+    return 0;
+}
+
+void sgmh_ctx_destroy(sgmh_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    DevBuf *bufs[] = {&c->img_l, &c->img_r, &c->disp, &c->disp1, &c->cl, &c->cr, &c->vol,
+                      &c->half_l, &c->half_r, &c->sob_l, &c->sob_r, &c->cl1, &c->cr1,
+                      &c->cost1, &c->cost2, &c->vol1};
+    for (DevBuf *b : bufs)
+        free_buf(*b);
+    for (hipEvent_t e : c->events)
+        (void)hipEventDestroy(e);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int sgmh_set_timing(sgmh_ctx *c, int on)
+{
+    REQUIRE(c, "sgmh_set_timing: NULL context");
+    c->timing = on != 0;
+    return 0;
+}
+
+int sgmh_get_times(sgmh_ctx *c, sgmh_times *t)
+{
+    REQUIRE(c && t, "sgmh_get_times: NULL argument");
+    double ph[PH_N];
+    for (int i = 0; i < PH_N; i++)
+        ph[i] = -1.0;
+    double total = 0.0;
+    if (!c->marks.empty()) {
+        CK(hipEventSynchronize(c->events[c->marks.back().second.second]));
+        for (auto &m : c->marks) {
+            float ms = 0.f;
+            CK(hipEventElapsedTime(&ms, c->events[m.second.first], c->events[m.second.second]));
+            ph[m.first] = (ph[m.first] < 0 ? 0.0 : ph[m.first]) + ms;
+        }
+        float all = 0.f;
+        CK(hipEventElapsedTime(&all, c->events[c->marks.front().second.first],
+                               c->events[c->marks.back().second.second]));
+        total = all;
+    }
+    t->census_ms = ph[PH_CENSUS];
+    t->cost_ms = ph[PH_COST];
+    t->aggregate_ms = ph[PH_AGG];
+    t->argmin_ms = ph[PH_ARGMIN];
+    t->transfer_ms = ph[PH_XFER];
+    t->total_ms = total;
+    return 0;
+}
+
+int sgmh_plain_dev(sgmh_ctx *c, const uint8_t *l, const uint8_t *r, int W, int H, int D, int paths,
+                   uint8_t *disp, void *stream)
+{
+    REQUIRE(c && l && r && disp, "sgmh_plain_dev: NULL argument");
+    if (check_dims(W, H, D))
+        return -1;
+    REQUIRE(paths == 4 || paths == 8, "sgmh_plain_dev: paths=%d (4 or 8)", paths);
+    CK(hipSetDevice(c->device));
+    hipStream_t s = pick(c, stream);
+    begin_call(c, s);
+    return plain_enqueue(c, s, l, r, W, H, D, paths, disp);
+}
+
+int sgmh_plain(sgmh_ctx *c, const uint8_t *left, const uint8_t *right, int W, int H, int D,
+               int paths, uint8_t *disp, sgmh_times *t)
+{
+    REQUIRE(c && left && right && disp, "sgmh_plain: NULL argument");
+    if (check_dims(W, H, D))
+        return -1;
+    REQUIRE(paths == 4 || paths == 8, "sgmh_plain: paths=%d (4 or 8)", paths);
+    CK(hipSetDevice(c->device));
+    const size_t N = (size_t)W * H;
+    if (ensure(c->img_l, N) || ensure(c->img_r, N) || ensure(c->disp, N))
+        return -1;
+    hipStream_t s = c->stream;
+    begin_call(c, s);
+    {
+        PhaseScope ps(c, s, PH_XFER);
+        CK(hipMemcpyAsync(c->img_l.p, left, N, hipMemcpyHostToDevice, s));
+        CK(hipMemcpyAsync(c->img_r.p, right, N, hipMemcpyHostToDevice, s));
+    }
+    if (plain_enqueue(c, s, (const uint8_t *)c->img_l.p, (const uint8_t *)c->img_r.p, W, H, D,
+                      paths, (uint8_t *)c->disp.p))
+        return -1;
+    {
+        PhaseScope ps(c, s, PH_XFER);
+        CK(hipMemcpyAsync(disp, c->disp.p, N, hipMemcpyDeviceToHost, s));
+    }
+    CK(hipStreamSynchronize(s));
+    if (t)
+        return sgmh_get_times(c, t);
+    return 0;
+}
+
+int sgmh_ms_dev(sgmh_ctx *c, const uint8_t *l, const uint8_t *r, int W, int H, int D, uint8_t *disp,
+                uint8_t *disp1, void *stream)
+{
+    REQUIRE(c && l && r && disp, "sgmh_ms_dev: NULL argument");
+    if (check_dims(W, H, D))
+        return -1;
+    REQUIRE(W % 2 == 0 && H % 2 == 0 && W >= 32 && H >= 32,
+            "sgmh_ms_dev: multiscale needs even W, H >= 32 (got %dx%d)", W, H);
+    CK(hipSetDevice(c->device));
+    hipStream_t s = pick(c, stream);
+    begin_call(c, s);
+    return ms_enqueue(c, s, l, r, W, H, D, disp, disp1);
+}
+
+int sgmh_ms(sgmh_ctx *c, const uint8_t *left, const uint8_t *right, int W, int H, int D,
+            uint8_t *disp, uint8_t *disp1, sgmh_times *t)
+{
+    REQUIRE(c && left && right && disp, "sgmh_ms: NULL argument");
+    if (check_dims(W, H, D))
+        return -1;
+    REQUIRE(W % 2 == 0 && H % 2 == 0 && W >= 32 && H >= 32,
+            "sgmh_ms: multiscale needs even W, H >= 32 (got %dx%d)", W, H);
+    CK(hipSetDevice(c->device));
+    const size_t N = (size_t)W * H, Nh = N / 4;
+    if (ensure(c->img_l, N) || ensure(c->img_r, N) || ensure(c->disp, N) ||
+        (disp1 && ensure(c->disp1, Nh)))
+        return -1;
+    hipStream_t s = c->stream;
+    begin_call(c, s);
+    {
+        PhaseScope ps(c, s, PH_XFER);
+        CK(hipMemcpyAsync(c->img_l.p, left, N, hipMemcpyHostToDevice, s));
+        CK(hipMemcpyAsync(c->img_r.p, right, N, hipMemcpyHostToDevice, s));
+    }
+    if (ms_enqueue(c, s, (const uint8_t *)c->img_l.p, (const uint8_t *)c->img_r.p, W, H, D,
+                   (uint8_t *)c->disp.p, disp1 ? (uint8_t *)c->disp1.p : nullptr))
+        return -1;
+    {
+        PhaseScope ps(c, s, PH_XFER);
+        CK(hipMemcpyAsync(disp, c->disp.p, Nh, hipMemcpyDeviceToHost, s));
+        if (disp1)
+            CK(hipMemcpyAsync(disp1, c->disp1.p, Nh, hipMemcpyDeviceToHost, s));
+    }
+    CK(hipStreamSynchronize(s));
+    if (t)
+        return sgmh_get_times(c, t);
+    return 0;
+}
+
+int sgmh_census_dev(sgmh_ctx *c, const uint8_t *img, int W, int H, uint64_t *codes, void *stream)
+{
+    REQUIRE(c && img && codes, "sgmh_census_dev: NULL argument");
+    REQUIRE(W >= 1 && H >= 1 && W <= 16384 && H <= 16384, "sgmh_census_dev: bad size %dx%d", W, H);
+    CK(hipSetDevice(c->device));
+    hipStream_t s = pick(c, stream);
+    begin_call(c, s);
+    return launch_census(s, img, codes, W, H, W);
+}
+
+int sgmh_sobel_dev(sgmh_ctx *c, const uint8_t *img, int W, int H, uint8_t *out, void *stream)
+{
+    REQUIRE(c && img && out, "sgmh_sobel_dev: NULL argument");
+    REQUIRE(W >= 1 && H >= 1 && W <= 16384 && H <= 16384, "sgmh_sobel_dev: bad size %dx%d", W, H);
+    CK(hipSetDevice(c->device));
+    hipStream_t s = pick(c, stream);
+    begin_call(c, s);
+    hipLaunchKernelGGL(k_sobel, dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, s, img, W, H, out);
+    CK(hipGetLastError());
+    return 0;
+}
+
+int sgmh_half_dev(sgmh_ctx *c, const uint8_t *img, int W, int H, uint8_t *out, void *stream)
+{
+    REQUIRE(c && img && out, "sgmh_half_dev: NULL argument");
+    REQUIRE(W >= 2 && H >= 2 && W <= 16384 && H <= 16384, "sgmh_half_dev: bad size %dx%d", W, H);
+    CK(hipSetDevice(c->device));
+    hipStream_t s = pick(c, stream);
+    begin_call(c, s);
+    hipLaunchKernelGGL(k_half, dim3((W / 2 + 63) / 64, (H / 2 + 3) / 4), dim3(256), 0, s, img, W, H, out);
+    CK(hipGetLastError());
+    return 0;
+}
+
+int sgmh_path_plain_dev(sgmh_ctx *c, const uint64_t *cl, const uint64_t *cr, int W, int H, int D,
+                        int dx, int dy, uint8_t *L, void *stream)
+{
+    REQUIRE(c && cl && cr && L, "sgmh_path_plain_dev: NULL argument");
+    if (check_dims(W, H, D))
+        return -1;
+    REQUIRE(dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && (dx | dy) != 0,
+            "sgmh_path_plain_dev: bad direction (%d,%d)", dx, dy);
+    CK(hipSetDevice(c->device));
+    hipStream_t s = pick(c, stream);
+    begin_call(c, s);
+    // re-pitch the caller's dense codes into the padded layout the kernels read
+    const size_t NC = census_padded_codes(W, H);
+    if (ensure(c->cl, NC * 8) || ensure(c->cr, NC * 8))
+        return -1;
+    uint64_t *pl = (uint64_t *)c->cl.p + kCensusPad, *pr = (uint64_t *)c->cr.p + kCensusPad;
+    const size_t pitch = (size_t)(W + kCensusPad) * 8;
+    CK(hipMemcpy2DAsync(pl, pitch, cl, (size_t)W * 8, (size_t)W * 8, H, hipMemcpyDeviceToDevice, s));
+    CK(hipMemcpy2DAsync(pr, pitch, cr, (size_t)W * 8, (size_t)W * 8, H, hipMemcpyDeviceToDevice, s));
+    const int dirs[1][3] = {{dx, dy, (int)kPlainP1}};
+    return plain_paths(c, s, pl, pr, W, H, D, dirs, 1, L, 0);
+}
+
+} // extern "C"

@@ -1,0 +1,161 @@
+// sgm_kernels.h -- device-side building blocks shared by the SGM kernels
+// (gfx950, wave64).
+//
+// Work layout used by every aggregation kernel: one scanline per 16-lane DPP
+// row, four scanlines per wave64.  Lane l of a row owns the DPL = D/16
+// consecutive disparities d = l*DPL + j (j = 0..DPL-1) in registers, so
+//   * the d-1 / d+1 neighbours inside a lane are registers,
+//   * the two that cross a lane boundary are one row_shr:1 / row_shl:1 DPP move,
+//   * min over all D is an in-lane min3 chain plus four DPP min steps
+//     (quad_perm x2, row_half_mirror, row_mirror) that leave the row minimum in
+//     every lane of the row: no LDS, no barrier, no readlane.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sgmk {
+
+constexpr int kRowLanes = 16;    // lanes per scanline (one DPP row)
+constexpr int kLinesPerWave = 4; // scanlines per wave64
+
+__device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+// DPP move with every lane valid (bound_ctrl irrelevant for these patterns).
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
+}
+
+// DPP move where lanes without a source keep `old`.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_or(uint32_t old, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+constexpr int kQuadSwap1 = 0xB1;      // quad_perm [1,0,3,2]
+constexpr int kQuadSwap2 = 0x4E;      // quad_perm [2,3,0,1]
+constexpr int kRowHalfMirror = 0x141; // lane i <- 7-i within each half row
+constexpr int kRowMirror = 0x140;     // lane i <- 15-i within the row
+constexpr int kRowShr1 = 0x111;       // lane i <- lane i-1 (lane 0 has none)
+constexpr int kRowShl1 = 0x101;       // lane i <- lane i+1 (lane 15 has none)
+
+// Minimum over the 16 lanes of this lane's DPP row, returned in every lane.
+__device__ __forceinline__ uint32_t row_min16(uint32_t v)
+{
+    v = umin(v, dpp<kQuadSwap1>(v));
+    v = umin(v, dpp<kQuadSwap2>(v));
+    v = umin(v, dpp<kRowHalfMirror>(v));
+    v = umin(v, dpp<kRowMirror>(v));
+    return v;
+}
+
+// One recurrence step for this lane's DPL disparities (common/sgm_params.h:50-55,
+// sgm_ref.c:61-72 and sgm_ms_ref.c:74-84):
+//   m(d) = min(Lp(d), Lp(d-1)+P1, Lp(d+1)+P1, mn+P2);  L(d) = C(d) + m(d) - mn
+// where mn = min_k Lp(k).  The reference saturates each bracketed add at 255;
+// since Lp(d) <= 255 is always one of the candidates, min() gives the same
+// result without the clamps.  Out-of-range neighbours (d = -1, d = D) are
+// absent, which the DPP `old` value 0xFFFFFFFF expresses.  SAT clamps the
+// final sum at 255 (multiscale rule; the plain rule cannot exceed 254).
+template <int DPL, bool SAT>
+__device__ __forceinline__ void sgm_step(uint32_t (&L)[DPL], const uint32_t (&C)[DPL],
+                                         uint32_t &mn, uint32_t P1, uint32_t P2)
+{
+    uint32_t a[DPL];
+#pragma unroll
+    for (int j = 0; j < DPL; j++)
+        a[j] = L[j] + P1;
+    const uint32_t below = dpp_or<kRowShr1>(0xFFFFFFFFu, a[DPL - 1]); // Lp(l*DPL-1)+P1
+    const uint32_t above = dpp_or<kRowShl1>(0xFFFFFFFFu, a[0]);       // Lp(l*DPL+DPL)+P1
+    const uint32_t jump = mn + P2;
+    const uint32_t neg = 0u - mn;
+    uint32_t lo = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < DPL; j++) {
+        const uint32_t nb_lo = j == 0 ? below : a[j - 1];
+        const uint32_t nb_hi = j == DPL - 1 ? above : a[j + 1];
+        uint32_t m = umin(umin(L[j], jump), umin(nb_lo, nb_hi));
+        uint32_t v = C[j] + m + neg;
+        if (SAT)
+            v = umin(v, 255u);
+        L[j] = v;
+        lo = umin(lo, v);
+    }
+    mn = row_min16(lo);
+}
+
+template <int DPL>
+__device__ __forceinline__ uint32_t lane_min(const uint32_t (&v)[DPL])
+{
+    uint32_t m = v[0];
+#pragma unroll
+    for (int j = 1; j < DPL; j++)
+        m = umin(m, v[j]);
+    return m;
+}
+
+// Pack DPL byte values (each < 256) and store them contiguously.
+template <int DPL>
+__device__ __forceinline__ void store_bytes(uint8_t *dst, const uint32_t (&v)[DPL])
+{
+    if constexpr (DPL == 1) {
+        *dst = (uint8_t)v[0];
+    } else if constexpr (DPL == 2) {
+        *reinterpret_cast<uint16_t *>(dst) = (uint16_t)(v[0] | (v[1] << 8));
+    } else {
+        static_assert(DPL % 4 == 0, "DPL must be 1, 2 or a multiple of 4");
+        uint32_t w[DPL / 4];
+#pragma unroll
+        for (int k = 0; k < DPL / 4; k++)
+            w[k] = v[4 * k] | (v[4 * k + 1] << 8) | (v[4 * k + 2] << 16) | (v[4 * k + 3] << 24);
+        if constexpr (DPL == 4) {
+            *reinterpret_cast<uint32_t *>(dst) = w[0];
+        } else if constexpr (DPL == 8) {
+            *reinterpret_cast<uint2 *>(dst) = make_uint2(w[0], w[1]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < DPL / 4; k += 4)
+                *reinterpret_cast<uint4 *>(dst + 4 * k) =
+                    make_uint4(w[k], w[k + 1], w[k + 2], w[k + 3]);
+        }
+    }
+}
+
+// Load DPL contiguous bytes and widen them.
+template <int DPL>
+__device__ __forceinline__ void load_bytes(const uint8_t *src, uint32_t (&v)[DPL])
+{
+    if constexpr (DPL == 1) {
+        v[0] = *src;
+    } else if constexpr (DPL == 2) {
+        const uint32_t h = *reinterpret_cast<const uint16_t *>(src);
+        v[0] = h & 0xFFu;
+        v[1] = h >> 8;
+    } else {
+        static_assert(DPL % 4 == 0, "DPL must be 1, 2 or a multiple of 4");
+        uint32_t w[DPL / 4];
+        if constexpr (DPL == 4) {
+            w[0] = *reinterpret_cast<const uint32_t *>(src);
+        } else if constexpr (DPL == 8) {
+            const uint2 q = *reinterpret_cast<const uint2 *>(src);
+            w[0] = q.x;
+            w[1] = q.y;
+        } else {
+#pragma unroll
+            for (int k = 0; k < DPL / 4; k += 4) {
+                const uint4 q = *reinterpret_cast<const uint4 *>(src + 4 * k);
+                w[k] = q.x;
+                w[k + 1] = q.y;
+                w[k + 2] = q.z;
+                w[k + 3] = q.w;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < DPL; j++)
+            v[j] = (w[j / 4] >> (8 * (j % 4))) & 0xFFu;
+    }
+}
+
+} // namespace sgmk

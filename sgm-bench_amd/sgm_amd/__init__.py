@@ -1,0 +1,273 @@
+"""sgm_amd -- Python binding of the MI355X SGM engine (libsgm_hip.so) and of its
+plain-C host helpers (libsgm_host.so), over ctypes.
+
+The engine itself is the C ABI in include/sgm_hip.h; this module only marshals
+numpy arrays (host buffers) or raw device pointers (for example from torch
+tensors) into it.  There is no fallback: if the shared libraries are missing,
+importing the GPU entry points raises, and every call that fails in the library
+raises SgmError with the library's own message.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(os.path.dirname(_HERE), "lib")
+REPO = os.path.dirname(os.path.dirname(_HERE))
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+
+
+class SgmError(RuntimeError):
+    pass
+
+
+def _load(name: str) -> ctypes.CDLL:
+    path = os.path.join(LIBDIR, name)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run `make -C sgm-bench_amd` "
+                          f"(or __graft_entry__.build()) first")
+    return ctypes.CDLL(path)
+
+
+_host: Optional[ctypes.CDLL] = None
+_hip: Optional[ctypes.CDLL] = None
+
+
+def host_lib() -> ctypes.CDLL:
+    global _host
+    if _host is None:
+        h = _load("libsgm_host.so")
+        h.sgm_scene_reference.argtypes = [_I, _I, _I, ctypes.c_uint32, _P, _P, _P]
+        h.sgm_scene_reference.restype = _I
+        h.sgm_scene_planes.argtypes = [ctypes.POINTER(SceneSpec), _P, _P, _P]
+        h.sgm_scene_planes.restype = _I
+        h.sgm_fnv1a64.argtypes = [_P, ctypes.c_size_t]
+        h.sgm_fnv1a64.restype = ctypes.c_uint64
+        h.sgm_pgm_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(_I), ctypes.POINTER(_I)]
+        h.sgm_pgm_read.restype = _P
+        h.sgm_pgm_write.argtypes = [ctypes.c_char_p, _P, _I, _I]
+        h.sgm_pgm_write.restype = _I
+        h.sgm_free.argtypes = [_P]
+        h.sgm_now_ms.restype = ctypes.c_double
+        _host = h
+    return _host
+
+
+class Times(ctypes.Structure):
+    """sgmh_times (include/sgm_hip.h)."""
+    _fields_ = [("census_ms", ctypes.c_double), ("cost_ms", ctypes.c_double),
+                ("aggregate_ms", ctypes.c_double), ("argmin_ms", ctypes.c_double),
+                ("transfer_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SceneSpec(ctypes.Structure):
+    """sgm_scene_spec (include/sgm_scene.h)."""
+    _fields_ = [("W", _I), ("H", _I), ("dmax", _I), ("seed", ctypes.c_uint32),
+                ("planes_lo", _I), ("planes_hi", _I), ("occl_permille", _I),
+                ("noise", _I), ("slanted", _I)]
+
+
+def hip_lib() -> ctypes.CDLL:
+    global _hip
+    if _hip is None:
+        h = _load("libsgm_hip.so")
+        h.sgmh_last_error.restype = ctypes.c_char_p
+        h.sgmh_ctx_create.argtypes = [ctypes.POINTER(_P), _I]
+        h.sgmh_ctx_destroy.argtypes = [_P]
+        h.sgmh_ctx_destroy.restype = None
+        h.sgmh_set_timing.argtypes = [_P, _I]
+        h.sgmh_get_times.argtypes = [_P, ctypes.POINTER(Times)]
+        h.sgmh_plain.argtypes = [_P, _P, _P, _I, _I, _I, _I, _P, ctypes.POINTER(Times)]
+        h.sgmh_plain_dev.argtypes = [_P, _P, _P, _I, _I, _I, _I, _P, _P]
+        h.sgmh_ms.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P, ctypes.POINTER(Times)]
+        h.sgmh_ms_dev.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P, _P]
+        h.sgmh_census_dev.argtypes = [_P, _P, _I, _I, _P, _P]
+        h.sgmh_sobel_dev.argtypes = [_P, _P, _I, _I, _P, _P]
+        h.sgmh_half_dev.argtypes = [_P, _P, _I, _I, _P, _P]
+        h.sgmh_path_plain_dev.argtypes = [_P, _P, _P, _I, _I, _I, _I, _I, _P, _P]
+        for name in ("sgmh_ctx_create", "sgmh_set_timing", "sgmh_get_times", "sgmh_plain",
+                     "sgmh_plain_dev", "sgmh_ms", "sgmh_ms_dev", "sgmh_census_dev",
+                     "sgmh_sobel_dev", "sgmh_half_dev", "sgmh_path_plain_dev"):
+            getattr(h, name).restype = _I
+        _hip = h
+    return _hip
+
+
+def _ptr(a: np.ndarray) -> _P:
+    return a.ctypes.data_as(_P)
+
+
+def _u8(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    return a
+
+
+def fnv1a64(a: np.ndarray) -> str:
+    """FNV-1a 64 of the bytes of `a`, as the reference prints it (%016llx)."""
+    a = np.ascontiguousarray(a)
+    return "%016x" % host_lib().sgm_fnv1a64(_ptr(a), a.nbytes)
+
+
+def scene_reference(W: int, H: int, D: int, seed: int) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """The reference's gen_synthetic scene (common/gen_synthetic.c), as (H, W) arrays."""
+    L = np.empty((H, W), np.uint8)
+    R = np.empty_like(L)
+    G = np.empty_like(L)
+    if host_lib().sgm_scene_reference(W, H, D, seed, _ptr(L), _ptr(R), _ptr(G)):
+        raise SgmError(f"sgm_scene_reference rejected {W}x{H} D={D}")
+    return L, R, G
+
+
+def scene_planes(W: int, H: int, dmax: int, seed: int, planes: Tuple[int, int] = (8, 12),
+                 occl_permille: int = 0, noise: int = 2, slanted: bool = False):
+    """Seeded piecewise-planar scene (include/sgm_scene.h), as (H, W) arrays."""
+    spec = SceneSpec(W, H, dmax, seed, planes[0], planes[1], occl_permille, noise, int(slanted))
+    L = np.empty((H, W), np.uint8)
+    R = np.empty_like(L)
+    G = np.empty_like(L)
+    if host_lib().sgm_scene_planes(ctypes.byref(spec), _ptr(L), _ptr(R), _ptr(G)):
+        raise SgmError("sgm_scene_planes rejected the spec")
+    return L, R, G
+
+
+# Named workloads: (W, H, D, generator kwargs).  The plain configurations are
+# 4-path; the multiscale ones 8-path two-scale.
+WORKLOADS = {
+    "P": dict(W=640, H=480, D=64, planes=(8, 12), occl_permille=0, seed=11),
+    "A": dict(W=1920, H=1080, D=64, planes=(20, 20), occl_permille=50, seed=21),
+    "B": dict(W=1920, H=800, D=64, planes=(20, 20), occl_permille=50, seed=31),
+    "C": dict(W=1920, H=1080, D=64, planes=(20, 20), occl_permille=50, seed=41),
+}
+
+
+def workload_scene(name: str):
+    w = WORKLOADS[name]
+    return scene_planes(w["W"], w["H"], w["D"] - 1, w["seed"], planes=w["planes"],
+                        occl_permille=w["occl_permille"], noise=2)
+
+
+def pgm_read(path: str) -> np.ndarray:
+    W, H = _I(), _I()
+    p = host_lib().sgm_pgm_read(path.encode(), ctypes.byref(W), ctypes.byref(H))
+    if not p:
+        raise SgmError(f"cannot read PGM {path}")
+    try:
+        return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint8)),
+                                     shape=(H.value, W.value)).copy()
+    finally:
+        host_lib().sgm_free(p)
+
+
+def pgm_write(path: str, img: np.ndarray) -> None:
+    img = _u8(img)
+    H, W = img.shape
+    if host_lib().sgm_pgm_write(path.encode(), _ptr(img), W, H):
+        raise SgmError(f"cannot write PGM {path}")
+
+
+class Context:
+    """One sgmh_ctx: a device, a stream, and the device workspaces."""
+
+    def __init__(self, device: int = 0):
+        self._lib = hip_lib()
+        h = _P()
+        if self._lib.sgmh_ctx_create(ctypes.byref(h), device):
+            raise SgmError(self.error())
+        self._h = h
+
+    def error(self) -> str:
+        return (self._lib.sgmh_last_error() or b"").decode()
+
+    def _ck(self, rc: int) -> None:
+        if rc:
+            raise SgmError(self.error())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sgmh_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ---- timing ----
+    def set_timing(self, on: bool) -> None:
+        self._ck(self._lib.sgmh_set_timing(self._h, int(on)))
+
+    def times(self) -> dict:
+        t = Times()
+        self._ck(self._lib.sgmh_get_times(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    # ---- host-buffer entry points ----
+    def plain(self, left: np.ndarray, right: np.ndarray, D: int = 64, paths: int = 4,
+              times: Optional[dict] = None) -> np.ndarray:
+        L, R = _u8(left), _u8(right)
+        H, W = L.shape
+        out = np.empty((H, W), np.uint8)
+        t = Times()
+        self._ck(self._lib.sgmh_plain(self._h, _ptr(L), _ptr(R), W, H, D, paths, _ptr(out),
+                                      ctypes.byref(t)))
+        if times is not None:
+            times.update(t.as_dict())
+        return out
+
+    def ms(self, left: np.ndarray, right: np.ndarray, D: int = 64, want_stage1: bool = False,
+           times: Optional[dict] = None):
+        L, R = _u8(left), _u8(right)
+        H, W = L.shape
+        out = np.empty((H // 2, W // 2), np.uint8)
+        out1 = np.empty((H // 2, W // 2), np.uint8) if want_stage1 else None
+        t = Times()
+        self._ck(self._lib.sgmh_ms(self._h, _ptr(L), _ptr(R), W, H, D, _ptr(out),
+                                   _ptr(out1) if out1 is not None else None, ctypes.byref(t)))
+        if times is not None:
+            times.update(t.as_dict())
+        return (out, out1) if want_stage1 else out
+
+    # ---- device-pointer entry points (ints are raw device addresses) ----
+    def plain_dev(self, d_left: int, d_right: int, W: int, H: int, D: int, paths: int,
+                  d_disp: int, stream: int = 0) -> None:
+        self._ck(self._lib.sgmh_plain_dev(self._h, d_left, d_right, W, H, D, paths, d_disp,
+                                          stream or None))
+
+    def ms_dev(self, d_left: int, d_right: int, W: int, H: int, D: int, d_disp: int,
+               d_disp1: int = 0, stream: int = 0) -> None:
+        self._ck(self._lib.sgmh_ms_dev(self._h, d_left, d_right, W, H, D, d_disp,
+                                       d_disp1 or None, stream or None))
+
+    def census_dev(self, d_img: int, W: int, H: int, d_codes: int, stream: int = 0) -> None:
+        self._ck(self._lib.sgmh_census_dev(self._h, d_img, W, H, d_codes, stream or None))
+
+    def sobel_dev(self, d_img: int, W: int, H: int, d_out: int, stream: int = 0) -> None:
+        self._ck(self._lib.sgmh_sobel_dev(self._h, d_img, W, H, d_out, stream or None))
+
+    def half_dev(self, d_img: int, W: int, H: int, d_out: int, stream: int = 0) -> None:
+        self._ck(self._lib.sgmh_half_dev(self._h, d_img, W, H, d_out, stream or None))
+
+    def path_plain_dev(self, d_cl: int, d_cr: int, W: int, H: int, D: int, dx: int, dy: int,
+                       d_L: int, stream: int = 0) -> None:
+        self._ck(self._lib.sgmh_path_plain_dev(self._h, d_cl, d_cr, W, H, D, dx, dy, d_L,
+                                               stream or None))

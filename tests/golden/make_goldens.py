@@ -1,0 +1,150 @@
+"""make_goldens.py -- write tests/golden/goldens.json, the hash fixtures that pin
+this repository's oracle and engine to the reference.
+
+Run in a container that has the reference at /root/reference (or
+$SGM_REFERENCE) after `oracle/build_ref.sh` built its binaries into
+oracle/_ref/.  Three kinds of record:
+
+  reference_goldens   FNV-1a 64 of the reference's own on-disk golden maps
+                      (data/golden, data/shared, data/multiscale), with the
+                      generator call that regenerates their input pair;
+  reference_inputs    sha256 of the reference generator's left/right PGM files
+                      for those calls (pins the generator restatement);
+  computed            FNV of the COMPILED REFERENCE oracle (sgm_ref /
+                      sgm_ms_ref built from its sources) on scenes from this
+                      repository's generators, at sizes and D the reference
+                      ships no golden for.
+
+Every input is regenerated from its recorded spec; only hashes are stored.
+"""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "sgm-bench_amd"))
+import sgm_amd  # noqa: E402
+
+REF = os.environ.get("SGM_REFERENCE", "/root/reference")
+BIN = os.path.join(ROOT, "oracle", "_ref")
+
+
+def fnv(a):
+    return sgm_amd.fnv1a64(np.ascontiguousarray(a, np.uint8))
+
+
+def run_plain(L, R, D, paths, tmp):
+    exe = os.path.join(BIN, "sgm_ref_d%d%s" % (D, "_p8" if paths == 8 else ""))
+    sgm_amd.pgm_write(os.path.join(tmp, "l.pgm"), L)
+    sgm_amd.pgm_write(os.path.join(tmp, "r.pgm"), R)
+    out = os.path.join(tmp, "o.pgm")
+    subprocess.run([exe, os.path.join(tmp, "l.pgm"), os.path.join(tmp, "r.pgm"), "-w", "0", "-n", "1",
+                    "-o", out, "--no-roofline"], check=True, capture_output=True, cwd=tmp)
+    return fnv(sgm_amd.pgm_read(out))
+
+
+def run_ms(L, R, tmp):
+    exe = os.path.join(BIN, "sgm_ms_ref_omp")
+    sgm_amd.pgm_write(os.path.join(tmp, "l.pgm"), L)
+    sgm_amd.pgm_write(os.path.join(tmp, "r.pgm"), R)
+    out = os.path.join(tmp, "o.pgm")
+    subprocess.run([exe, os.path.join(tmp, "l.pgm"), os.path.join(tmp, "r.pgm"), "-o", out, "-n", "1"],
+                   check=True, capture_output=True, cwd=tmp)
+    return fnv(sgm_amd.pgm_read(out))
+
+
+def scene(spec):
+    if spec["kind"] == "reference":
+        L, R, _ = sgm_amd.scene_reference(spec["W"], spec["H"], spec["gen_D"], spec["seed"])
+    else:
+        L, R, _ = sgm_amd.scene_planes(spec["W"], spec["H"], spec["dmax"], spec["seed"],
+                                       planes=tuple(spec["planes"]), occl_permille=spec["occl"],
+                                       noise=spec["noise"], slanted=bool(spec["slanted"]))
+    return L, R
+
+
+def ref_scene(W, H, D, seed):
+    return {"kind": "reference", "W": W, "H": H, "gen_D": D, "seed": seed}
+
+
+def planes(W, H, dmax, seed, planes=(8, 12), occl=0, noise=2, slanted=0):
+    return {"kind": "planes", "W": W, "H": H, "dmax": dmax, "seed": seed, "planes": list(planes),
+            "occl": occl, "noise": noise, "slanted": slanted}
+
+
+def main():
+    if not os.path.isdir(BIN):
+        sys.exit("oracle/_ref missing: run oracle/build_ref.sh first")
+    out = {"reference_goldens": [], "reference_inputs": [], "computed": []}
+    # on-disk reference goldens and the generator calls that make their inputs
+    disk = [
+        ("A_synthetic", ref_scene(1920, 1080, 64, 1), "plain", 64, 4, "data/golden/synthetic.pgm"),
+        ("shared_d64", ref_scene(1920, 1080, 256, 3), "plain", 64, 4, "data/shared/golden_d64.pgm"),
+        ("shared_d128", ref_scene(1920, 1080, 256, 3), "plain", 128, 4, "data/shared/golden_d128.pgm"),
+        ("shared_d256", ref_scene(1920, 1080, 256, 3), "plain", 256, 4, "data/shared/golden_d256.pgm"),
+        ("B_multiscale", ref_scene(1920, 800, 64, 1), "ms", 64, 8, "data/multiscale/golden_ms.pgm"),
+        ("C_multiscale", ref_scene(1920, 1080, 64, 1), "ms", 64, 8, "data/multiscale/golden_ms_1080.pgm"),
+    ]
+    for name, spec, kind, D, paths, path in disk:
+        g = sgm_amd.pgm_read(os.path.join(REF, path))
+        out["reference_goldens"].append({"name": name, "scene": spec, "mode": kind, "D": D,
+                                         "paths": paths, "shape": list(g.shape), "fnv": fnv(g),
+                                         "source": path})
+    with tempfile.TemporaryDirectory() as tmp:
+        for (W, H, D, seed) in [(1920, 1080, 64, 1), (1920, 1080, 256, 3), (1920, 800, 64, 1)]:
+            subprocess.run([os.path.join(BIN, "gen_synthetic_d%d" % D), str(W), str(H), tmp, str(seed)],
+                           check=True, capture_output=True)
+            rec = {"scene": ref_scene(W, H, D, seed)}
+            for n in ("left", "right", "gt"):
+                rec[n + "_sha256"] = hashlib.sha256(open(os.path.join(tmp, n + ".pgm"), "rb").read()).hexdigest()
+            out["reference_inputs"].append(rec)
+
+        computed = [
+            # plain, the configurations and the batch's plain members
+            ("P_640x480_d64", planes(640, 480, 63, 11), "plain", 64, 4),
+            ("A_1920x1080_d64", planes(1920, 1080, 63, 21, (20, 20), 50), "plain", 64, 4),
+            ("A_1920x1080_d128", planes(1920, 1080, 127, 22, (20, 20), 50), "plain", 128, 4),
+            ("batch_1280x720_d64", planes(1280, 720, 63, 101, (8, 12)), "plain", 64, 4),
+            ("batch_1920x1080_d128", planes(1920, 1080, 127, 103, (10, 30), 50), "plain", 128, 4),
+            ("learn_1920x1080_d128", planes(1920, 1080, 127, 7001, (10, 30), 0, 2, 1), "plain", 128, 4),
+            # the reference generator at other sizes / D
+            ("ref_640x480_d64", ref_scene(640, 480, 64, 1), "plain", 64, 4),
+            ("ref_640x480_d128", ref_scene(640, 480, 128, 1), "plain", 128, 4),
+            ("ref_1280x720_d64", ref_scene(1280, 720, 64, 1), "plain", 64, 4),
+            ("ref_960x540_d128", ref_scene(960, 540, 128, 1), "plain", 128, 4),
+            # 8 paths (SGM_PATHS=8 in the reference oracle)
+            ("p8_640x480_d64", planes(640, 480, 63, 12), "plain", 64, 8),
+            ("p8_ref_640x480_d64", ref_scene(640, 480, 64, 1), "plain", 64, 8),
+            ("p8_1920x1080_d64", planes(1920, 1080, 63, 23, (20, 20), 50), "plain", 64, 8),
+            # small, ragged and extreme shapes
+            ("odd_97x61_d64", planes(97, 61, 63, 5, (3, 5)), "plain", 64, 4),
+            ("odd_97x61_d64_p8", planes(97, 61, 63, 5, (3, 5)), "plain", 64, 8),
+            ("tiny_16x16_d16", planes(16, 16, 15, 6, (1, 2)), "plain", 16, 4),
+            ("tiny_33x17_d32", planes(33, 17, 31, 8, (1, 3)), "plain", 32, 4),
+            ("narrow_40x300_d64", planes(40, 300, 39, 9, (2, 4)), "plain", 64, 4),
+            ("wide_1000x20_d256", planes(1000, 20, 255, 10, (4, 6)), "plain", 256, 4),
+            ("d256_640x480", planes(640, 480, 255, 13, (8, 12)), "plain", 256, 4),
+            # multiscale (the reference oracle is D=64 only)
+            ("B_1920x800_ms", planes(1920, 800, 63, 31, (20, 20), 50), "ms", 64, 8),
+            ("C_1920x1080_ms", planes(1920, 1080, 63, 41, (20, 20), 50), "ms", 64, 8),
+            ("ms_ref_640x480", ref_scene(640, 480, 64, 1), "ms", 64, 8),
+            ("ms_odd_98x62", planes(98, 62, 63, 14, (3, 5)), "ms", 64, 8),
+        ]
+        for name, spec, kind, D, paths in computed:
+            L, R = scene(spec)
+            h = run_plain(L, R, D, paths, tmp) if kind == "plain" else run_ms(L, R, tmp)
+            out["computed"].append({"name": name, "scene": spec, "mode": kind, "D": D, "paths": paths,
+                                    "input_fnv": fnv(np.concatenate([L.ravel(), R.ravel()])),
+                                    "fnv": h})
+            print(name, h, flush=True)
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "goldens.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

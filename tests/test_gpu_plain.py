@@ -1,0 +1,116 @@
+"""GPU parity of the plain path: every phase and the whole pipeline against the
+CPU oracle, and against the reference's golden hashes, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle
+import sgm_amd
+from conftest import make_scene, rng_image
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+@pytest.mark.parametrize("H,W", [(16, 16), (61, 97), (480, 640), (1080, 1920)])
+def test_census_descriptors(ctx, H, W):
+    img = rng_image(H, W, W + H)
+    d_img = dev(img)
+    d_codes = torch.empty((H, W), dtype=torch.int64, device="cuda")
+    ctx.census_dev(d_img.data_ptr(), W, H, d_codes.data_ptr(), stream())
+    torch.cuda.synchronize()
+    got = d_codes.cpu().numpy().view(np.uint64)
+    want = oracle.census(img)
+    diff = np.argwhere(got != want)
+    assert diff.size == 0, f"first differing descriptor at {diff[0]}: got {got[tuple(diff[0])]:x} want {want[tuple(diff[0])]:x}"
+
+
+DIRS = [(1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, 1), (1, -1), (-1, -1)]
+
+
+@pytest.mark.parametrize("D", [16, 32, 64, 128, 256])
+@pytest.mark.parametrize("H,W", [(23, 77), (40, 300)])
+def test_path_volumes(ctx, D, H, W):
+    L, R = rng_image(H, W, 7 * D + W), rng_image(H, W, 7 * D + W + 1)
+    cl, cr = oracle.census(L), oracle.census(R)
+    C = oracle.cost(cl, cr, D)
+    d_cl, d_cr = dev(cl.view(np.int64)), dev(cr.view(np.int64))
+    d_L = torch.empty((H, W, D), dtype=torch.uint8, device="cuda")
+    for dx, dy in DIRS:
+        d_L.fill_(0xAB)
+        ctx.path_plain_dev(d_cl.data_ptr(), d_cr.data_ptr(), W, H, D, dx, dy, d_L.data_ptr(), stream())
+        torch.cuda.synchronize()
+        got = d_L.cpu().numpy()
+        want = oracle.path_L(C, dx, dy)
+        bad = np.argwhere(got != want)
+        assert bad.size == 0, f"dir ({dx},{dy}) D={D}: first mismatch at {bad[0]}"
+
+
+@pytest.mark.parametrize("H,W,D,paths", [
+    (16, 16, 16, 4), (17, 33, 32, 4), (61, 97, 64, 4), (61, 97, 64, 8), (300, 40, 64, 4),
+    (120, 200, 128, 4), (120, 200, 128, 8), (20, 1000, 256, 4), (64, 64, 256, 8)])
+def test_plain_matches_oracle(ctx, H, W, D, paths):
+    L, R, _ = sgm_amd.scene_planes(W, H, min(D - 1, W - 1), W * H + D, planes=(3, 6))
+    got = ctx.plain(L, R, D, paths)
+    want = oracle.sgm_plain(L, R, D, paths)
+    bad = np.argwhere(got != want)
+    assert bad.size == 0, f"{bad.shape[0]} pixels differ, first at {bad[0]}"
+
+
+def test_plain_dev_matches_host_entry(ctx):
+    L, R = make_scene({"kind": "reference", "W": 640, "H": 480, "gen_D": 64, "seed": 1})
+    host = ctx.plain(L, R, 64, 4)
+    d_l, d_r = dev(L), dev(R)
+    d_o = torch.empty_like(d_l)
+    ctx.plain_dev(d_l.data_ptr(), d_r.data_ptr(), 640, 480, 64, 4, d_o.data_ptr(), stream())
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(d_o.cpu().numpy(), host)
+
+
+def test_reference_goldens_plain(ctx, goldens):
+    """The reference's own on-disk goldens, input regenerated on the fly."""
+    for g in goldens["reference_goldens"]:
+        if g["mode"] != "plain":
+            continue
+        L, R = make_scene(g["scene"])
+        out = ctx.plain(L, R, g["D"], g["paths"])
+        assert sgm_amd.fnv1a64(out) == g["fnv"], g["name"]
+
+
+def test_computed_goldens_plain(ctx, goldens):
+    """Hashes of the compiled reference oracle on this repository's scenes."""
+    for g in goldens["computed"]:
+        if g["mode"] != "plain":
+            continue
+        L, R = make_scene(g["scene"])
+        assert sgm_amd.fnv1a64(np.concatenate([L.ravel(), R.ravel()])) == g["input_fnv"], g["name"]
+        out = ctx.plain(L, R, g["D"], g["paths"])
+        assert sgm_amd.fnv1a64(out) == g["fnv"], g["name"]
+
+
+def test_repeat_and_resize(ctx):
+    """Workspaces are reused across calls and regrown on a size change."""
+    a = sgm_amd.scene_planes(300, 200, 63, 1)
+    b = sgm_amd.scene_planes(700, 500, 63, 2)
+    ra = oracle.sgm_plain(a[0], a[1], 64, 4)
+    rb = oracle.sgm_plain(b[0], b[1], 64, 4)
+    for L, R, want in ((a[0], a[1], ra), (b[0], b[1], rb), (a[0], a[1], ra), (a[0], a[1], ra)):
+        np.testing.assert_array_equal(ctx.plain(L, R, 64, 4), want)
+
+
+def test_errors_are_reported(ctx):
+    L = np.zeros((32, 32), np.uint8)
+    with pytest.raises(sgm_amd.SgmError, match="D=48"):
+        ctx.plain(L, L, 48, 4)
+    with pytest.raises(sgm_amd.SgmError, match="paths"):
+        ctx.plain(L, L, 64, 6)
+    with pytest.raises(sgm_amd.SgmError, match="outside"):
+        ctx.plain(np.zeros((8, 8), np.uint8), np.zeros((8, 8), np.uint8), 64, 4)

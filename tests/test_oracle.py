@@ -1,0 +1,222 @@
+"""The CPU oracle is pinned here against the reference's goldens, the compiled
+reference's hashes, and independent formulations of each step (never by
+retyping its own formula)."""
+import itertools
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import sgm_amd
+from conftest import make_scene, rng_image
+
+
+# ---------------------------------------------------------------- census
+def census_bruteforce(img):
+    """numpy restatement by shifted-image comparison (a different algorithm
+    from the oracle's per-pixel loop)."""
+    H, W = img.shape
+    pad = np.pad(img, ((3, 3), (4, 4)), mode="edge").astype(np.int32)
+    c = img.astype(np.int32)
+    code = np.zeros((H, W), np.uint64)
+    n = 0
+    for dy in range(-3, 4):
+        for dx in range(-4, 5):
+            if dx == 0 and dy == 0:
+                continue
+            nb = pad[3 + dy:3 + dy + H, 4 + dx:4 + dx + W]
+            code |= (nb < c).astype(np.uint64) << np.uint64(n)
+            n += 1
+    assert n == 62
+    return code
+
+
+@pytest.mark.parametrize("H,W", [(1, 1), (7, 9), (13, 40), (61, 97)])
+def test_census_matches_bruteforce(H, W):
+    img = rng_image(H, W, H * 1000 + W)
+    np.testing.assert_array_equal(oracle.census(img), census_bruteforce(img))
+
+
+def test_census_bit_order():
+    """bit 0 is the neighbour at (dx,dy)=(-4,-3), bit 61 at (+4,+3); the centre
+    is skipped, so (+1,0) is bit 31 (sgm_params.h:45-47)."""
+    for (dx, dy, bit) in [(-4, -3, 0), (4, 3, 61), (-1, 0, 30), (1, 0, 31), (0, -3, 4)]:
+        img = np.full((21, 21), 100, np.uint8)
+        img[10 + dy, 10 + dx] = 50
+        code = int(oracle.census(img)[10, 10])
+        assert code == 1 << bit, (dx, dy, hex(code))
+
+
+def test_census_edge_replication():
+    img = rng_image(9, 12, 3)
+    # clamped coordinates == census of the edge-padded image, cropped
+    big = np.pad(img, ((5, 5), (6, 6)), mode="edge")
+    np.testing.assert_array_equal(oracle.census(img), oracle.census(big)[5:-5, 6:-6])
+
+
+# ---------------------------------------------------------------- cost
+def test_cost_closed_form():
+    cl = np.random.default_rng(1).integers(0, 2**62, size=(5, 70), dtype=np.uint64)
+    cr = np.random.default_rng(2).integers(0, 2**62, size=(5, 70), dtype=np.uint64)
+    C = oracle.cost(cl, cr, 64, invalid=62)
+    for y, x, d in itertools.product(range(5), range(0, 70, 7), range(0, 64, 5)):
+        want = 62 if x - d < 0 else bin(int(cl[y, x]) ^ int(cr[y, x - d])).count("1")
+        assert C[y, x, d] == want
+    C63 = oracle.cost(cl, cr, 64, invalid=63)
+    assert C63[0, 0, 5] == 63
+
+
+# ---------------------------------------------------------------- step
+def step_transition_matrix(prev, c, P1, P2, sat):
+    """L(d) = C(d) + min_d' sat(Lp(d') + pen(d, d')) - min Lp, pen 0 / P1 / P2:
+    the textbook transition form of the SGM recurrence."""
+    D = prev.size
+    d = np.arange(D)
+    dist = np.abs(d[:, None] - d[None, :])
+    pen = np.where(dist == 0, 0, np.where(dist == 1, P1, P2))
+    cand = np.minimum(prev[None, :].astype(np.int64) + pen, 255)
+    m = cand.min(axis=1)
+    v = c.astype(np.int64) + m - prev.min()
+    if sat:
+        v = np.minimum(v, 255)
+    return v.astype(np.uint8)
+
+
+@pytest.mark.parametrize("sat,P1,P2", [(False, 20, 192), (True, 40, 200), (True, 10, 200)])
+def test_step_exhaustive_small(sat, P1, P2):
+    rng = np.random.default_rng(P1 + P2)
+    for D in (1, 2, 3, 16):
+        for _ in range(400):
+            prev = rng.integers(0, 256, D).astype(np.uint8)
+            if rng.random() < 0.3:
+                prev[rng.integers(0, D)] = 255
+            if rng.random() < 0.3:
+                prev[:] = prev.min()
+            cmax = 62 if not sat else 63
+            c = rng.integers(0, cmax + 1, D).astype(np.uint8)
+            if not sat:
+                # plain-path operating range: Lp - min Lp <= P2 keeps L <= 254
+                prev = np.minimum(prev, int(prev.min()) + P2).astype(np.uint8)
+            np.testing.assert_array_equal(oracle.step(prev, c, P1, P2, sat),
+                                          step_transition_matrix(prev, c, P1, P2, sat))
+
+
+# ---------------------------------------------------------------- paths
+def path_bruteforce(C, dx, dy, P1, P2, sat):
+    H, W, D = C.shape
+    L = np.zeros_like(C)
+    ys = range(H - 1, -1, -1) if dy < 0 else range(H)
+    xs = range(W - 1, -1, -1) if dx < 0 else range(W)
+    for y in ys:
+        for x in xs:
+            px, py = x - dx, y - dy
+            if 0 <= px < W and 0 <= py < H:
+                L[y, x] = step_transition_matrix(L[py, px], C[y, x], P1, P2, sat)
+            else:
+                L[y, x] = C[y, x]
+    return L
+
+
+@pytest.mark.parametrize("dx,dy", [(1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, 1), (1, -1), (-1, -1)])
+def test_path_L_matches_transition_form(dx, dy):
+    img_l, img_r = rng_image(9, 23, 5), rng_image(9, 23, 6)
+    C = oracle.cost(oracle.census(img_l), oracle.census(img_r), 16)
+    np.testing.assert_array_equal(oracle.path_L(C, dx, dy), path_bruteforce(C, dx, dy, 20, 192, False))
+    np.testing.assert_array_equal(oracle.path_L(C, dx, dy, 10, 200, True),
+                                  path_bruteforce(C, dx, dy, 10, 200, True))
+
+
+def test_argmin_lowest_d_wins_ties():
+    S = np.full((1, 3, 8), 100, np.uint16)
+    S[0, 0, [2, 5]] = 7
+    S[0, 1, [7]] = 1
+    S[0, 2, :] = 9
+    np.testing.assert_array_equal(oracle.argmin(S), [[2, 7, 0]])
+
+
+def test_plain_scalar_equals_omp():
+    L, R = rng_image(50, 90, 1), rng_image(50, 90, 2)
+    for paths in (4, 8):
+        np.testing.assert_array_equal(oracle.sgm_plain(L, R, 32, paths, threads=1),
+                                      oracle.sgm_plain(L, R, 32, paths, threads=4))
+
+
+def test_plain_single_direction_consistency():
+    """With one direction the disparity is the argmin of that direction's L."""
+    L, R = rng_image(20, 70, 3), rng_image(20, 70, 4)
+    C = oracle.cost(oracle.census(L), oracle.census(R), 64)
+    Lv = oracle.path_L(C, 1, 0)
+    np.testing.assert_array_equal(oracle.argmin(Lv.astype(np.uint16)), np.argmin(Lv, axis=2))
+
+
+# ---------------------------------------------------------------- ms pieces
+def test_sobel_and_half_closed_forms():
+    img = rng_image(11, 17, 9).astype(np.int64)
+    p = np.pad(img, 1, mode="edge")
+    gx = (p[:-2, 2:] - p[:-2, :-2]) + 2 * (p[1:-1, 2:] - p[1:-1, :-2]) + (p[2:, 2:] - p[2:, :-2])
+    want = np.clip(128 + (gx >> 3), 0, 255)   # >> on negative int64 is arithmetic
+    np.testing.assert_array_equal(oracle.sobelx(img.astype(np.uint8)), want)
+    img = rng_image(10, 16, 8).astype(np.int64)
+    want = (img[0::2, 0::2] + img[0::2, 1::2] + img[1::2, 0::2] + img[1::2, 1::2] + 2) >> 2
+    np.testing.assert_array_equal(oracle.half(img.astype(np.uint8)), want)
+
+
+# ---------------------------------------------------------------- goldens
+def test_generator_matches_reference_inputs(goldens):
+    import hashlib
+    for rec in goldens["reference_inputs"]:
+        sp = rec["scene"]
+        L, R, G = sgm_amd.scene_reference(sp["W"], sp["H"], sp["gen_D"], sp["seed"])
+        hdr = b"P5\n%d %d\n255\n" % (sp["W"], sp["H"])
+        for name, arr in (("left", L), ("right", R), ("gt", G)):
+            assert hashlib.sha256(hdr + arr.tobytes()).hexdigest() == rec[name + "_sha256"], (sp, name)
+
+
+@pytest.mark.slow
+def test_oracle_reproduces_reference_goldens(goldens):
+    for g in goldens["reference_goldens"]:
+        L, R = make_scene(g["scene"])
+        if g["mode"] == "plain":
+            out = oracle.sgm_plain(L, R, g["D"], g["paths"])
+        else:
+            out = oracle.sgm_ms(L, R, g["D"])
+        assert list(out.shape) == g["shape"]
+        assert oracle.fnv1a64(out) == g["fnv"], g["name"]
+
+
+def test_oracle_matches_compiled_reference(goldens):
+    for g in goldens["computed"]:
+        sp = g["scene"]
+        if sp["W"] * sp["H"] * g["D"] > 640 * 480 * 64 * (2 if g["mode"] == "plain" else 1):
+            continue  # full-size cases: test_oracle_matches_compiled_reference_full
+        L, R = make_scene(sp)
+        assert sgm_amd.fnv1a64(np.concatenate([L.ravel(), R.ravel()])) == g["input_fnv"], g["name"]
+        out = oracle.sgm_plain(L, R, g["D"], g["paths"]) if g["mode"] == "plain" else oracle.sgm_ms(L, R, g["D"])
+        assert oracle.fnv1a64(out) == g["fnv"], g["name"]
+
+
+@pytest.mark.slow
+def test_oracle_matches_compiled_reference_full(goldens):
+    for g in goldens["computed"]:
+        sp = g["scene"]
+        if sp["W"] * sp["H"] * g["D"] <= 640 * 480 * 64 * (2 if g["mode"] == "plain" else 1):
+            continue
+        L, R = make_scene(sp)
+        out = oracle.sgm_plain(L, R, g["D"], g["paths"]) if g["mode"] == "plain" else oracle.sgm_ms(L, R, g["D"])
+        assert oracle.fnv1a64(out) == g["fnv"], g["name"]
+
+
+def test_golden_discriminates(goldens):
+    """The discrimination rule of the reference's checker
+    (scripts/check_golden_discriminates.py:64-119): max disparity D-1 present,
+    >= 8 distinct values, >= 0.5% of pixels in the top decile."""
+    from sgm_amd.goldencheck import discriminates
+    g = next(x for x in goldens["computed"] if x["name"] == "P_640x480_d64")
+    L, R = make_scene(g["scene"])
+    ok, stats = discriminates(oracle.sgm_plain(L, R, 64, 4), 64)
+    assert ok, stats
+    # a truncated search (d < 48) must fail it
+    ok_trunc, _ = discriminates(np.minimum(oracle.sgm_plain(L, R, 64, 4), 47), 64)
+    assert not ok_trunc
