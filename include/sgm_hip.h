@@ -77,7 +77,8 @@ int sgmh_get_times(sgmh_ctx *ctx, sgmh_times *t);
 int sgmh_plain(sgmh_ctx *ctx, const uint8_t *left, const uint8_t *right,
                int W, int H, int D, int paths, uint8_t *disp, sgmh_times *t);
 
-/* Device buffers, asynchronous on `stream` (NULL: the context's stream). */
+/* Device buffers, asynchronous on `stream` (a hipStream_t; NULL is the HIP
+ * default stream, as in the HIP API). */
 int sgmh_plain_dev(sgmh_ctx *ctx, const uint8_t *d_left, const uint8_t *d_right,
                    int W, int H, int D, int paths, uint8_t *d_disp, void *stream);
 

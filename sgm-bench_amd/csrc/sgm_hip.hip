@@ -211,11 +211,90 @@ __global__ __launch_bounds__(256) void k_cost(const uint64_t *__restrict__ cl,
 // ---------------------------------------------------------------------------
 // path aggregation: all directions of a frame in one launch
 // ---------------------------------------------------------------------------
-// Census codes handed to the aggregation kernels are row-padded: row y starts
-// at codes + y*cstride and kCensusPad codes of slack precede every row, so a
-// lane may read cr[y][x-d] for any d < 256 without a bounds branch (the value
-// read for x-d < 0 is discarded by the invalid-cost select).
-constexpr int kCensusPad = 256;
+// Memory goes through raw buffer resources: a load outside [0, num_records)
+// returns 0 and a store there is dropped by the hardware.  The step loops are
+// therefore straight-line code -- steps past the end of a scanline store to
+// offset 0xFFFFFFFF, census reads left of x=0 come back as harmless values the
+// invalid-cost select discards -- which lets the compiler count outstanding
+// memory operations exactly and keep the prefetch ring in flight.
+// Every volume a kernel addresses must stay below 2^31 bytes (checked on the
+// host).
+
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000; // raw, untyped buffer (gfx9 family)
+constexpr uint32_t kDrop = 0xFFFFFFFFu;  // an offset no resource covers
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, kRsrcWord3);
+}
+
+__device__ __forceinline__ uint64_t ld64(rsrc_t r, uint32_t off)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    return ((uint64_t)v[1] << 32) | v[0];
+}
+
+// Store DPL packed bytes at byte offset `off` (kDrop: no store).
+template <int DPL>
+__device__ __forceinline__ void st_bytes(rsrc_t r, uint32_t off, const uint32_t (&v)[DPL])
+{
+    if constexpr (DPL == 1) {
+        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v[0], r, off, 0, 0);
+    } else if constexpr (DPL == 2) {
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(v[0] | (v[1] << 8)), r, off, 0, 0);
+    } else {
+        uint32_t w[DPL / 4];
+#pragma unroll
+        for (int k = 0; k < DPL / 4; k++)
+            w[k] = v[4 * k] | (v[4 * k + 1] << 8) | (v[4 * k + 2] << 16) | (v[4 * k + 3] << 24);
+        if constexpr (DPL == 4) {
+            __builtin_amdgcn_raw_buffer_store_b32(w[0], r, off, 0, 0);
+        } else if constexpr (DPL == 8) {
+            typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+            const v2u q = {w[0], w[1]};
+            __builtin_amdgcn_raw_buffer_store_b64(q, r, off, 0, 0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < DPL / 4; k += 4) {
+                typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+                const v4u q = {w[k], w[k + 1], w[k + 2], w[k + 3]};
+                __builtin_amdgcn_raw_buffer_store_b128(q, r, off == kDrop ? kDrop : off + 4 * k, 0, 0);
+            }
+        }
+    }
+}
+
+// Load DPL bytes at byte offset `off` as NW = max(1, DPL/4) words.
+template <int DPL>
+struct CostWords {
+    static constexpr int NW = DPL < 4 ? 1 : DPL / 4;
+    uint32_t w[NW];
+};
+template <int DPL>
+__device__ __forceinline__ void ld_cost(rsrc_t r, uint32_t off, CostWords<DPL> &c)
+{
+    if constexpr (DPL == 1) {
+        c.w[0] = __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+    } else if constexpr (DPL == 2) {
+        c.w[0] = __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+    } else if constexpr (DPL == 4) {
+        c.w[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    } else if constexpr (DPL == 8) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        c.w[0] = v[0];
+        c.w[1] = v[1];
+    } else {
+#pragma unroll
+        for (int k = 0; k < DPL / 4; k += 4) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 4 * k, 0, 0);
+            c.w[k] = v[0];
+            c.w[k + 1] = v[1];
+            c.w[k + 2] = v[2];
+            c.w[k + 3] = v[3];
+        }
+    }
+}
 
 struct DirTask {
     int dx, dy, P1;
@@ -224,11 +303,18 @@ struct DirTask {
     uint8_t *L;      // output volume of this direction
 };
 
+// Census buffers read by the path kernels carry kLead codes of slack before
+// pixel 0, so a disparity window that starts left of x = 0 on row 0 still has
+// a non-negative offset (a load straddling offset 0 would otherwise be out of
+// range as a whole and lose its valid half).
+constexpr int kLead = 256;
+
 struct PathArgs {
-    const uint64_t *cl, *cr; // padded census codes (cost computed on the fly)
+    const uint64_t *cl, *cr; // census codes; pixel i at index kLead + i
     const uint8_t *cost;     // or a materialised cost volume
-    int W, H, D, cstride;
+    int W, H, D;
     uint32_t P2, invalid;
+    uint32_t out_bytes;      // bytes of each direction's output volume
     int ndirs;
     DirTask dir[8];
 };
@@ -273,174 +359,187 @@ __device__ __forceinline__ uint64_t dpp64_shl1(uint64_t old, uint64_t v)
     return ((uint64_t)hi << 32) | lo;
 }
 
-template <int DPL, bool SAT>
-__device__ __forceinline__ void advance(uint32_t (&L)[DPL], const uint32_t (&C)[DPL], uint32_t &mn,
-                                        bool first, uint32_t P1, uint32_t P2)
-{
-    if (first) {
-#pragma unroll
-        for (int j = 0; j < DPL; j++)
-            L[j] = C[j];
-        mn = row_min16(lane_min<DPL>(L));
-    } else {
-        sgm_step<DPL, SAT>(L, C, mn, P1, P2);
-    }
-}
-
-// Horizontal direction, cost from census codes.  Lane l keeps the window
-// w[j] = cr[y][x - l*DPL - j] in registers; one step along the row shifts the
-// window by one disparity: the code leaving lane l's slice enters lane l+1's
-// (L->R) or lane l-1's (R->L) through one row_shr:1 / row_shl:1 DPP move, and
-// the end lane takes the one new code of the step.  The window lives in a
-// register ring indexed by the step modulo DPL, so the shift costs no moves.
-template <int DPL, bool SAT>
+// Horizontal direction DX (+1: L->R, -1: R->L), cost from census codes.
+// Lane l keeps the window w[j] = cr[y][x - l*DPL - j] in registers; one step
+// along the row shifts it by one disparity: the code leaving lane l's slice
+// enters lane l+1's (L->R) or lane l-1's (R->L) through one row_shr:1 /
+// row_shl:1 DPP move, and the end lane takes the one new code of the step.
+// The window is a register ring indexed by the step modulo DPL (PF is a
+// multiple of DPL), so the shift itself moves nothing.
+// The path starts from Lp = 0, mn = 0: with P1 > 0 that step yields L = C
+// exactly, the reference's first-pixel rule (sgm_ref.c:90-91), branch-free.
+template <int DPL, bool SAT, int DX>
 __device__ __forceinline__ void path_horizontal(const PathArgs &a, const DirTask &t, LineGeom ln,
                                                 int l, int maxsteps)
 {
-    constexpr int PF = DPL > 8 ? DPL : 8; // prefetch distance in steps; a multiple of DPL
-    const int W = a.W, D = a.D, dx = t.dx;
+    constexpr int PF = DPL > 8 ? DPL : 8;
+    const int W = a.W, D = a.D;
     const uint32_t P1 = (uint32_t)t.P1, P2 = a.P2, inval = a.invalid;
-    const bool live = ln.steps > 0;
-    const int y = live ? ln.y : 0;
-    const uint64_t *clrow = a.cl + (size_t)y * a.cstride;
-    const uint64_t *crrow = a.cr + (size_t)y * a.cstride;
-    uint8_t *out = t.L + ((size_t)y * W + ln.x) * D + l * DPL;
-    const long ostep = (long)dx * D;
+    const uint32_t nb = (uint32_t)(a.W * a.H + kLead) * 8u;
+    const rsrc_t rl = make_rsrc(a.cl, nb), rr = make_rsrc(a.cr, nb);
+    const rsrc_t ro = make_rsrc(t.L, a.out_bytes);
     const int lbase = l * DPL;
-
-    // phys[(j + s) mod DPL] holds logical w[j] at step s for R->L; for L->R the
-    // logical index runs the other way: phys[(j - s) mod DPL].  PF is a
-    // multiple of DPL, so s mod DPL == k mod DPL inside the unrolled block.
-    uint64_t phys[DPL];
+    const int row = ln.y * W + kLead;           // code index of (0, y)
+    const int orow = ln.y * W;                  // pixel index of (0, y)
+    const int steps = ln.steps;
     int x = ln.x;
+
+    // window one step before the start
+    uint64_t phys[DPL];
 #pragma unroll
-    for (int j = 0; j < DPL; j++)
-        phys[j] = crrow[x - lbase - j];
-    uint64_t pl[PF], pn[PF]; // prefetched cl[x_s] and the new window code of step s
+    for (int j = 0; j < DPL; j++) {
+        const int slot = DX > 0 ? (j + 1) % DPL : (j - 1 + DPL) % DPL;
+        phys[slot] = ld64(rr, (uint32_t)(row + x - DX - lbase - j) * 8u);
+    }
+    // prefetched cl[x_s] and the window code entering at step s
+    const int enter = DX > 0 ? 0 : -(D - 1);
+    uint64_t pl[PF], pn[PF];
 #pragma unroll
     for (int k = 0; k < PF; k++) {
-        const int xs = x + k * dx;
-        const bool ok = live && k < ln.steps;
-        pl[k] = ok ? clrow[xs] : 0;
-        pn[k] = ok ? crrow[dx > 0 ? xs : xs - (D - 1)] : 0;
+        pl[k] = ld64(rl, (uint32_t)(row + x + k * DX) * 8u);
+        pn[k] = ld64(rr, (uint32_t)(row + x + k * DX + enter) * 8u);
+        __builtin_amdgcn_sched_barrier(0); // slot 0 first: its wait is the shortest
     }
     uint32_t L[DPL], mn = 0;
+#pragma unroll
+    for (int j = 0; j < DPL; j++)
+        L[j] = 0;
+    uint32_t off = (uint32_t)(orow + x) * (uint32_t)D + (uint32_t)lbase;
+    const uint32_t ostep = (uint32_t)(DX * D);
     for (int s0 = 0; s0 < maxsteps; s0 += PF) {
 #pragma unroll
         for (int k = 0; k < PF; k++) {
             const int s = s0 + k;
-            if (s < ln.steps) {
-                if (s > 0) {
-                    // shift the window by one step
-                    if (dx > 0) {
-                        const int slot = ((-k) % DPL + DPL) % DPL; // logical 0 at step s
-                        phys[slot] = dpp64_shr1(pn[k], phys[slot]);    // was logical DPL-1
-                    } else {
-                        const int slot = ((k - 1) % DPL + DPL) % DPL; // logical DPL-1 now
-                        phys[slot] = dpp64_shl1(pn[k], phys[slot]);       // was logical 0
-                    }
-                }
-                const uint64_t cL = pl[k];
-                const int xs = x;
-                // prefetch step s + PF
-                {
-                    const int sp = s + PF;
-                    const bool ok = sp < ln.steps;
-                    const int xp = xs + PF * dx;
-                    pl[k] = ok ? clrow[xp] : 0;
-                    pn[k] = ok ? crrow[dx > 0 ? xp : xp - (D - 1)] : 0;
-                }
-                uint32_t C[DPL];
-#pragma unroll
-                for (int j = 0; j < DPL; j++) {
-                    const int slot = dx > 0 ? (((j - k) % DPL) + DPL) % DPL : (j + k) % DPL;
-                    const uint32_t h = (uint32_t)__popcll(cL ^ phys[slot]);
-                    C[j] = xs - lbase - j >= 0 ? h : inval;
-                }
-                advance<DPL, SAT>(L, C, mn, s == 0, P1, P2);
-                store_bytes<DPL>(out, L);
-                out += ostep;
-                x += dx;
+            // shift the window to step s
+            if constexpr (DX > 0) {
+                const int slot = ((-k) % DPL + DPL) % DPL;        // logical 0 at step s
+                phys[slot] = dpp64_shr1(pn[k], phys[slot]);      // was logical DPL-1
+            } else {
+                const int slot = ((k - 1) % DPL + DPL) % DPL;     // logical DPL-1 at step s
+                phys[slot] = dpp64_shl1(pn[k], phys[slot]);      // was logical 0
             }
+            const uint64_t cL = pl[k];
+            uint32_t C[DPL];
+            const int rel = x - lbase;
+#pragma unroll
+            for (int j = 0; j < DPL; j++) {
+                const int slot = DX > 0 ? (((j - k) % DPL) + DPL) % DPL : (j + k) % DPL;
+                const uint32_t h = (uint32_t)__popcll(cL ^ phys[slot]);
+                C[j] = rel >= j ? h : inval;
+            }
+            // ring slot k is dead now: refill it for step s + PF in place
+            __builtin_amdgcn_sched_barrier(0);
+            const int xp = x + PF * DX;
+            pl[k] = ld64(rl, (uint32_t)(row + xp) * 8u);
+            pn[k] = ld64(rr, (uint32_t)(row + xp + enter) * 8u);
+            sgm_step<DPL, SAT>(L, C, mn, P1, P2);
+            st_bytes<DPL>(ro, s < steps ? off : kDrop, L);
+            off += ostep;
+            x += DX;
+            // keep each step's uses of the ring in program order, so a
+            // prefetched value is consumed PF steps after its load was issued
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
 
-// Any direction: cost from census codes read per lane, or from a materialised
-// cost volume, with a prefetch ring PF steps deep.  DECIM stores L only at
-// even (x, y), into a (W/2)x(H/2) layout (the multiscale output needs S only
-// there, sgm_ms_ref.c:194-199).
+// Any direction: cost from census codes read per lane (FROM_CENSUS), or from a
+// materialised uint8 cost volume, with a prefetch ring PF steps deep.  DECIM
+// stores L only at even (x, y), into a (W/2)x(H/2) layout (the multiscale
+// argmin reads S only there, sgm_ms_ref.c:194-199).
 template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM>
 __device__ __forceinline__ void path_generic(const PathArgs &a, const DirTask &t, LineGeom ln, int l,
                                              int maxsteps)
 {
-    constexpr int PF = DPL <= 4 ? 4 : 2;
-    constexpr int NV = FROM_CENSUS ? DPL + 1 : (DPL + 3) / 4; // 64- or 32-bit words per step
-    using word = typename std::conditional<FROM_CENSUS, uint64_t, uint32_t>::type;
+    constexpr int PF = DPL <= 4 ? 8 : (DPL <= 8 ? 4 : 2);
     const int W = a.W, D = a.D, dx = t.dx, dy = t.dy;
     const uint32_t P1 = (uint32_t)t.P1, P2 = a.P2, inval = a.invalid;
     const int lbase = l * DPL;
     const int hw = W >> 1;
-    int x = ln.x, y = ln.y;
+    const int steps = ln.steps;
+    const rsrc_t ro = make_rsrc(t.L, a.out_bytes);
+    rsrc_t rl, rr, rc;
+    if constexpr (FROM_CENSUS) {
+        const uint32_t nb = (uint32_t)(W * a.H + kLead) * 8u;
+        rl = make_rsrc(a.cl, nb);
+        rr = make_rsrc(a.cr, nb);
+    } else {
+        rc = make_rsrc(a.cost, (uint32_t)(W * a.H) * (uint32_t)D);
+    }
+    const int pstep = dy * W + dx;
+    int x = ln.x;
+    int pix = ln.y * W + ln.x;
+    int y = ln.y;
 
-    auto fetch = [&](int xs, int ys, word (&v)[NV]) {
+    // ring[k] holds the inputs of step s with s % PF == k
+    constexpr int NQ = FROM_CENSUS ? DPL + 1 : CostWords<DPL>::NW;
+    uint64_t rq[PF][FROM_CENSUS ? DPL + 1 : 1];
+    CostWords<DPL> rw[PF];
+    auto fetch = [&](int p, int k) {
         if constexpr (FROM_CENSUS) {
-            const uint64_t *cr = a.cr + (size_t)ys * a.cstride + xs - lbase;
-            v[DPL] = a.cl[(size_t)ys * a.cstride + xs];
-#pragma unroll
-            for (int j = 0; j < DPL; j++)
-                v[j] = cr[-j];
-        } else {
-            const uint8_t *p = a.cost + ((size_t)ys * W + xs) * D + lbase;
+            rq[k][DPL] = ld64(rl, (uint32_t)(p + kLead) * 8u);
+            // the lane's DPL window codes cr[p - lbase - j] are contiguous and
+            // descending in j: fetch them 16 bytes at a time
+            const uint32_t lo = (uint32_t)(p + kLead - lbase - (DPL - 1)) * 8u;
             if constexpr (DPL == 1) {
-                v[0] = *p;
-            } else if constexpr (DPL == 2) {
-                v[0] = *reinterpret_cast<const uint16_t *>(p);
+                rq[k][0] = ld64(rr, lo);
             } else {
 #pragma unroll
-                for (int q = 0; q < NV; q++)
-                    v[q] = reinterpret_cast<const uint32_t *>(p)[q];
+                for (int i = 0; i < DPL; i += 2) {
+                    const auto q = __builtin_amdgcn_raw_buffer_load_b128(rr, lo + 8u * i, 0, 0);
+                    rq[k][DPL - 1 - i] = ((uint64_t)q[1] << 32) | q[0];
+                    rq[k][DPL - 2 - i] = ((uint64_t)q[3] << 32) | q[2];
+                }
             }
+        } else {
+            ld_cost<DPL>(rc, (uint32_t)p * (uint32_t)D + (uint32_t)lbase, rw[k]);
         }
     };
-
-    word ring[PF][NV];
+    (void)NQ;
 #pragma unroll
     for (int k = 0; k < PF; k++) {
-        if (k < ln.steps)
-            fetch(x + k * dx, y + k * dy, ring[k]);
+        fetch(pix + k * pstep, k);
+        __builtin_amdgcn_sched_barrier(0); // slot 0 first: its wait is the shortest
     }
     uint32_t L[DPL], mn = 0;
+#pragma unroll
+    for (int j = 0; j < DPL; j++)
+        L[j] = 0;
     for (int s0 = 0; s0 < maxsteps; s0 += PF) {
 #pragma unroll
         for (int k = 0; k < PF; k++) {
             const int s = s0 + k;
-            if (s < ln.steps) {
-                uint32_t C[DPL];
-                if constexpr (FROM_CENSUS) {
-                    const uint64_t cL = ring[k][DPL];
+            uint32_t C[DPL];
+            if constexpr (FROM_CENSUS) {
+                const uint64_t cL = rq[k][DPL];
+                const int rel = x - lbase;
 #pragma unroll
-                    for (int j = 0; j < DPL; j++) {
-                        const uint32_t h = (uint32_t)__popcll(cL ^ ring[k][j]);
-                        C[j] = x - lbase - j >= 0 ? h : inval;
-                    }
-                } else {
+                for (int j = 0; j < DPL; j++) {
+                    const uint32_t h = (uint32_t)__popcll(cL ^ rq[k][j]);
+                    C[j] = rel >= j ? h : inval;
+                }
+            } else {
 #pragma unroll
-                    for (int j = 0; j < DPL; j++)
-                        C[j] = (ring[k][j / 4] >> (8 * (j % 4))) & 0xFFu;
-                }
-                if (s + PF < ln.steps)
-                    fetch(x + PF * dx, y + PF * dy, ring[k]);
-                advance<DPL, SAT>(L, C, mn, s == 0, P1, P2);
-                if constexpr (!DECIM) {
-                    store_bytes<DPL>(t.L + ((size_t)y * W + x) * D + lbase, L);
-                } else {
-                    if (((x | y) & 1) == 0)
-                        store_bytes<DPL>(t.L + ((size_t)(y >> 1) * hw + (x >> 1)) * D + lbase, L);
-                }
-                x += dx;
-                y += dy;
+                for (int j = 0; j < DPL; j++)
+                    C[j] = (rw[k].w[j / 4] >> (8 * (j % 4))) & 0xFFu;
             }
+            // ring slot k is dead now: refill it for step s + PF in place
+            __builtin_amdgcn_sched_barrier(0);
+            fetch(pix + PF * pstep, k);
+            sgm_step<DPL, SAT>(L, C, mn, P1, P2);
+            uint32_t off;
+            if constexpr (!DECIM) {
+                off = (uint32_t)pix * (uint32_t)D + (uint32_t)lbase;
+            } else {
+                off = (((x | y) & 1) == 0)
+                          ? (uint32_t)((y >> 1) * hw + (x >> 1)) * (uint32_t)D + (uint32_t)lbase
+                          : kDrop;
+            }
+            st_bytes<DPL>(ro, s < steps ? off : kDrop, L);
+            pix += pstep;
+            x += dx;
+            y += dy;
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
@@ -465,7 +564,10 @@ __global__ __launch_bounds__(64) void k_path(PathArgs a)
         ln = line_geom(line, t.dx, t.dy, a.W, a.H);
     if constexpr (FROM_CENSUS && !DECIM) {
         if (t.dy == 0) {
-            path_horizontal<DPL, SAT>(a, t, ln, l, maxsteps);
+            if (t.dx > 0)
+                path_horizontal<DPL, SAT, 1>(a, t, ln, l, maxsteps);
+            else
+                path_horizontal<DPL, SAT, -1>(a, t, ln, l, maxsteps);
             return;
         }
     }
@@ -634,6 +736,11 @@ static int launch_path_t(hipStream_t s, const PathArgs &a, int blocks)
 static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndirs,
                         bool from_census, bool sat, uint8_t *vol, size_t dir_stride, int decimate)
 {
+    // every byte offset the kernels form must fit the 32-bit buffer offsets
+    const size_t in_bytes = (size_t)a.W * a.H * (from_census ? 8 : (size_t)a.D);
+    REQUIRE(dir_stride < (1ull << 31) && in_bytes < (1ull << 31),
+            "sgmh: %dx%d D=%d exceeds the 2 GiB per-volume limit", a.W, a.H, a.D);
+    a.out_bytes = (uint32_t)dir_stride;
     int order[8];
     for (int k = 0; k < ndirs; k++)
         order[k] = k;
@@ -696,14 +803,13 @@ static int launch_cost(hipStream_t s, const uint64_t *cl, const uint64_t *cr, in
     return 0;
 }
 
+// Device-buffer entry points run on the caller's stream; NULL is the HIP
+// default stream, as in the HIP API itself.
 static hipStream_t pick(sgmh_ctx *ctx, void *stream)
 {
-    return stream ? (hipStream_t)stream : ctx->stream;
+    (void)ctx;
+    return (hipStream_t)stream;
 }
-
-// Padded census layout for the aggregation kernels: kCensusPad codes before
-// row 0 and a row stride of W + kCensusPad.
-static size_t census_padded_codes(int W, int H) { return (size_t)kCensusPad + (size_t)H * (W + kCensusPad); }
 
 static int plain_paths(sgmh_ctx *ctx, hipStream_t s, const uint64_t *cl, const uint64_t *cr, int W,
                        int H, int D, const int (*dirs)[3], int ndirs, uint8_t *vol, size_t dir_stride)
@@ -715,7 +821,6 @@ static int plain_paths(sgmh_ctx *ctx, hipStream_t s, const uint64_t *cl, const u
     a.W = W;
     a.H = H;
     a.D = D;
-    a.cstride = W + kCensusPad;
     a.P2 = kPlainP2;
     a.invalid = kPlainInvalid;
     return launch_paths(s, a, dirs, ndirs, true, false, vol, dir_stride, 0);
@@ -724,15 +829,15 @@ static int plain_paths(sgmh_ctx *ctx, hipStream_t s, const uint64_t *cl, const u
 static int plain_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W,
                          int H, int D, int paths, uint8_t *disp)
 {
-    const size_t N = (size_t)W * H, NC = census_padded_codes(W, H);
-    if (ensure(ctx->cl, NC * 8) || ensure(ctx->cr, NC * 8) || ensure(ctx->vol, N * D * paths))
+    const size_t N = (size_t)W * H;
+    if (ensure(ctx->cl, (N + kLead) * 8) || ensure(ctx->cr, (N + kLead) * 8) ||
+        ensure(ctx->vol, N * D * paths))
         return -1;
-    uint64_t *cl = (uint64_t *)ctx->cl.p + kCensusPad, *cr = (uint64_t *)ctx->cr.p + kCensusPad;
-    const int cstride = W + kCensusPad;
+    uint64_t *cl = (uint64_t *)ctx->cl.p, *cr = (uint64_t *)ctx->cr.p; // padded bases
     uint8_t *vol = (uint8_t *)ctx->vol.p;
     {
         PhaseScope ps(ctx, s, PH_CENSUS);
-        if (launch_census(s, l, cl, W, H, cstride) || launch_census(s, r, cr, W, H, cstride))
+        if (launch_census(s, l, cl + kLead, W, H, W) || launch_census(s, r, cr + kLead, W, H, W))
             return -1;
     }
     {
@@ -1038,16 +1143,15 @@ int sgmh_path_plain_dev(sgmh_ctx *c, const uint64_t *cl, const uint64_t *cr, int
     CK(hipSetDevice(c->device));
     hipStream_t s = pick(c, stream);
     begin_call(c, s);
-    // re-pitch the caller's dense codes into the padded layout the kernels read
-    const size_t NC = census_padded_codes(W, H);
-    if (ensure(c->cl, NC * 8) || ensure(c->cr, NC * 8))
+    // copy the caller's dense codes behind the lead pad the kernels expect
+    const size_t N = (size_t)W * H;
+    if (ensure(c->cl, (N + kLead) * 8) || ensure(c->cr, (N + kLead) * 8))
         return -1;
-    uint64_t *pl = (uint64_t *)c->cl.p + kCensusPad, *pr = (uint64_t *)c->cr.p + kCensusPad;
-    const size_t pitch = (size_t)(W + kCensusPad) * 8;
-    CK(hipMemcpy2DAsync(pl, pitch, cl, (size_t)W * 8, (size_t)W * 8, H, hipMemcpyDeviceToDevice, s));
-    CK(hipMemcpy2DAsync(pr, pitch, cr, (size_t)W * 8, (size_t)W * 8, H, hipMemcpyDeviceToDevice, s));
+    uint64_t *pl = (uint64_t *)c->cl.p, *pr = (uint64_t *)c->cr.p;
+    CK(hipMemcpyAsync(pl + kLead, cl, N * 8, hipMemcpyDeviceToDevice, s));
+    CK(hipMemcpyAsync(pr + kLead, cr, N * 8, hipMemcpyDeviceToDevice, s));
     const int dirs[1][3] = {{dx, dy, (int)kPlainP1}};
-    return plain_paths(c, s, pl, pr, W, H, D, dirs, 1, L, 0);
+    return plain_paths(c, s, pl, pr, W, H, D, dirs, 1, L, N * D);
 }
 
 } // extern "C"

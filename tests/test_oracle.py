@@ -98,7 +98,7 @@ def test_step_exhaustive_small(sat, P1, P2):
             c = rng.integers(0, cmax + 1, D).astype(np.uint8)
             if not sat:
                 # plain-path operating range: Lp - min Lp <= P2 keeps L <= 254
-                prev = np.minimum(prev, int(prev.min()) + P2).astype(np.uint8)
+                prev = np.minimum(prev.astype(np.int64), int(prev.min()) + P2).astype(np.uint8)
             np.testing.assert_array_equal(oracle.step(prev, c, P1, P2, sat),
                                           step_transition_matrix(prev, c, P1, P2, sat))
 
