@@ -18,6 +18,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "sgm_hip.h"
@@ -373,6 +374,10 @@ __device__ __forceinline__ void path_horizontal(const PathArgs &a, const DirTask
                                                 int l, int maxsteps)
 {
     constexpr int PF = DPL > 8 ? DPL : 8;
+    // Rows are the longest scanlines of the frame and set its critical path:
+    // give their waves issue priority over the column / diagonal waves that
+    // share their SIMD.
+    __builtin_amdgcn_s_setprio(2);
     const int W = a.W, D = a.D;
     const uint32_t P1 = (uint32_t)t.P1, P2 = a.P2, inval = a.invalid;
     const uint32_t nb = (uint32_t)(a.W * a.H + kLead) * 8u;
@@ -404,9 +409,15 @@ __device__ __forceinline__ void path_horizontal(const PathArgs &a, const DirTask
 #pragma unroll
     for (int j = 0; j < DPL; j++)
         L[j] = 0;
-    uint32_t off = (uint32_t)(orow + x) * (uint32_t)D + (uint32_t)lbase;
-    const uint32_t ostep = (uint32_t)(DX * D);
-    for (int s0 = 0; s0 < maxsteps; s0 += PF) {
+    // a group past the last line stores nothing: its offset stays at kDrop
+    uint32_t off = steps > 0 ? (uint32_t)(orow + x) * (uint32_t)D + (uint32_t)lbase : kDrop;
+    const uint32_t ostep = steps > 0 ? (uint32_t)(DX * D) : 0u;
+
+    // One block of PF steps.  CHECK: some step of the block may have x-d < 0
+    // for a lane's disparities, or run past the end of the line.  Blocks
+    // without either (all but ~D/PF + 1 of them) skip both selects.
+    auto block = [&](int s0, auto check_tag) {
+        constexpr bool CHECK = decltype(check_tag)::value;
 #pragma unroll
         for (int k = 0; k < PF; k++) {
             const int s = s0 + k;
@@ -425,7 +436,7 @@ __device__ __forceinline__ void path_horizontal(const PathArgs &a, const DirTask
             for (int j = 0; j < DPL; j++) {
                 const int slot = DX > 0 ? (((j - k) % DPL) + DPL) % DPL : (j + k) % DPL;
                 const uint32_t h = (uint32_t)__popcll(cL ^ phys[slot]);
-                C[j] = rel >= j ? h : inval;
+                C[j] = CHECK ? (rel >= j ? h : inval) : h;
             }
             // ring slot k is dead now: refill it for step s + PF in place
             __builtin_amdgcn_sched_barrier(0);
@@ -433,21 +444,38 @@ __device__ __forceinline__ void path_horizontal(const PathArgs &a, const DirTask
             pl[k] = ld64(rl, (uint32_t)(row + xp) * 8u);
             pn[k] = ld64(rr, (uint32_t)(row + xp + enter) * 8u);
             sgm_step<DPL, SAT>(L, C, mn, P1, P2);
-            st_bytes<DPL>(ro, s < steps ? off : kDrop, L);
+            st_bytes<DPL>(ro, (!CHECK || s < steps) ? off : kDrop, L);
             off += ostep;
             x += DX;
             // keep each step's uses of the ring in program order, so a
             // prefetched value is consumed PF steps after its load was issued
             __builtin_amdgcn_sched_barrier(0);
         }
+    };
+    // blocks [0, lo) and [hi, maxsteps) need the selects (lo, hi multiples of PF)
+    // (every live row has maxsteps = W steps, so the bounds are wave-uniform)
+    int lo, hi;
+    if (DX > 0) {
+        lo = ((D - 1) + PF - 1) / PF * PF;  // x = s < D-1 somewhere in the block
+        hi = maxsteps / PF * PF;            // a block reaching past the line end
+    } else {
+        lo = 0;
+        hi = std::min(maxsteps / PF, (W - D + 1) / PF) * PF; // x = W-1-s >= D-1 throughout
     }
+    int s0 = 0;
+    for (; s0 < lo && s0 < maxsteps; s0 += PF)
+        block(s0, std::true_type{});
+    for (; s0 < hi && s0 < maxsteps; s0 += PF)
+        block(s0, std::false_type{});
+    for (; s0 < maxsteps; s0 += PF)
+        block(s0, std::true_type{});
 }
 
 // Any direction: cost from census codes read per lane (FROM_CENSUS), or from a
 // materialised uint8 cost volume, with a prefetch ring PF steps deep.  DECIM
 // stores L only at even (x, y), into a (W/2)x(H/2) layout (the multiscale
 // argmin reads S only there, sgm_ms_ref.c:194-199).
-template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM>
+template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, bool VERT>
 __device__ __forceinline__ void path_generic(const PathArgs &a, const DirTask &t, LineGeom ln, int l,
                                              int maxsteps)
 {
@@ -470,9 +498,17 @@ __device__ __forceinline__ void path_generic(const PathArgs &a, const DirTask &t
     int x = ln.x;
     int pix = ln.y * W + ln.x;
     int y = ln.y;
+    // a group past the last line stores nothing
+    const uint32_t dead = steps > 0 ? 0u : kDrop;
+    // On a column x is fixed, so which disparities have x-d < 0 is fixed too:
+    // cost = max(hamming, inv[j]) with inv = invalid there and 0 elsewhere
+    // (the invalid cost 62 is at least any Hamming distance of 62-bit codes).
+    uint32_t inv[DPL];
+#pragma unroll
+    for (int j = 0; j < DPL; j++)
+        inv[j] = x - lbase >= j ? 0u : inval;
 
     // ring[k] holds the inputs of step s with s % PF == k
-    constexpr int NQ = FROM_CENSUS ? DPL + 1 : CostWords<DPL>::NW;
     uint64_t rq[PF][FROM_CENSUS ? DPL + 1 : 1];
     CostWords<DPL> rw[PF];
     auto fetch = [&](int p, int k) {
@@ -495,7 +531,6 @@ __device__ __forceinline__ void path_generic(const PathArgs &a, const DirTask &t
             ld_cost<DPL>(rc, (uint32_t)p * (uint32_t)D + (uint32_t)lbase, rw[k]);
         }
     };
-    (void)NQ;
 #pragma unroll
     for (int k = 0; k < PF; k++) {
         fetch(pix + k * pstep, k);
@@ -505,7 +540,11 @@ __device__ __forceinline__ void path_generic(const PathArgs &a, const DirTask &t
 #pragma unroll
     for (int j = 0; j < DPL; j++)
         L[j] = 0;
-    for (int s0 = 0; s0 < maxsteps; s0 += PF) {
+
+    // One block of PF steps; CHECK: a step of the block may lie past the end
+    // of its line (always possible on diagonals, whose lines differ in length).
+    auto block = [&](int s0, auto check_tag) {
+        constexpr bool CHECK = decltype(check_tag)::value;
 #pragma unroll
         for (int k = 0; k < PF; k++) {
             const int s = s0 + k;
@@ -516,7 +555,10 @@ __device__ __forceinline__ void path_generic(const PathArgs &a, const DirTask &t
 #pragma unroll
                 for (int j = 0; j < DPL; j++) {
                     const uint32_t h = (uint32_t)__popcll(cL ^ rq[k][j]);
-                    C[j] = rel >= j ? h : inval;
+                    if constexpr (VERT)
+                        C[j] = umin(255u, h > inv[j] ? h : inv[j]);
+                    else
+                        C[j] = rel >= j ? h : inval;
                 }
             } else {
 #pragma unroll
@@ -535,13 +577,23 @@ __device__ __forceinline__ void path_generic(const PathArgs &a, const DirTask &t
                           ? (uint32_t)((y >> 1) * hw + (x >> 1)) * (uint32_t)D + (uint32_t)lbase
                           : kDrop;
             }
-            st_bytes<DPL>(ro, s < steps ? off : kDrop, L);
+            if constexpr (CHECK)
+                off = s < steps ? off : kDrop;
+            st_bytes<DPL>(ro, off | dead, L);
             pix += pstep;
             x += dx;
             y += dy;
             __builtin_amdgcn_sched_barrier(0);
         }
-    }
+    };
+    // axis-aligned lines all have maxsteps steps: only a final partial block
+    // needs the end-of-line select
+    const int full = (dx == 0 || dy == 0) ? maxsteps / PF * PF : 0;
+    int s0 = 0;
+    for (; s0 < full; s0 += PF)
+        block(s0, std::false_type{});
+    for (; s0 < maxsteps; s0 += PF)
+        block(s0, std::true_type{});
 }
 
 template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM>
@@ -571,39 +623,82 @@ __global__ __launch_bounds__(64) void k_path(PathArgs a)
             return;
         }
     }
-    path_generic<DPL, FROM_CENSUS, SAT, DECIM>(a, t, ln, l, maxsteps);
+    if (FROM_CENSUS && t.dx == 0)
+        path_generic<DPL, FROM_CENSUS, SAT, DECIM, true>(a, t, ln, l, maxsteps);
+    else
+        path_generic<DPL, FROM_CENSUS, SAT, DECIM, false>(a, t, ln, l, maxsteps);
 }
 
 // ---------------------------------------------------------------------------
 // S = sum over directions, argmin (lowest d wins ties), sgm_ref.c:117-124
 // ---------------------------------------------------------------------------
-template <int DPL>
-__global__ __launch_bounds__(256) void k_wta(const uint8_t *__restrict__ L, size_t dir_stride,
-                                             int ndirs, size_t npix, int D, uint8_t *__restrict__ disp)
+// A lane owns 16 consecutive disparities of one pixel (one 16-byte load per
+// direction), so a pixel spans LPP = D/16 lanes and a wave covers 64/LPP
+// pixels.  Sums stay packed two per 32-bit word (S <= 8*255 fits 16 bits):
+// even bytes in one word, odd bytes in another.  U batches of pixels are
+// loaded before any is reduced, to keep enough bytes in flight per wave.
+template <int LPP>
+__device__ __forceinline__ uint32_t group_min(uint32_t v)
 {
-    const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
-    const size_t stride = (size_t)gridDim.x * 16;
-    for (size_t p = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + g; p < npix; p += stride) {
-        uint32_t s[DPL];
+    if constexpr (LPP >= 2)
+        v = umin(v, dpp<kQuadSwap1>(v));
+    if constexpr (LPP >= 4)
+        v = umin(v, dpp<kQuadSwap2>(v));
+    if constexpr (LPP >= 8)
+        v = umin(v, dpp<kRowHalfMirror>(v));
+    if constexpr (LPP >= 16)
+        v = umin(v, dpp<kRowMirror>(v));
+    return v;
+}
+
+template <int LPP, int NDIRS>
+__global__ __launch_bounds__(256) void k_wta(const uint8_t *__restrict__ L, size_t dir_stride,
+                                             int npix, uint8_t *__restrict__ disp)
+{
+    constexpr int D = 16 * LPP, PPW = 64 / LPP, U = 2;
+    const int lane = threadIdx.x & 63, sub = lane % LPP, slot = lane / LPP;
+    const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int nwaves = (int)gridDim.x * 4;
+    for (int b0 = wave * PPW * U; b0 < npix; b0 += nwaves * PPW * U) {
+        uint4 v[U][8];
+        int pix[U];
 #pragma unroll
-        for (int j = 0; j < DPL; j++)
-            s[j] = 0;
-        const uint8_t *src = L + p * D + l * DPL;
-        for (int k = 0; k < ndirs; k++) {
-            uint32_t v[DPL];
-            load_bytes<DPL>(src + k * dir_stride, v);
+        for (int u = 0; u < U; u++) {
+            const int p = b0 + u * PPW + slot;
+            pix[u] = p;
+            const int pc = p < npix ? p : npix - 1;
+            const uint8_t *src = L + (size_t)pc * D + sub * 16;
 #pragma unroll
-            for (int j = 0; j < DPL; j++)
-                s[j] += v[j];
+            for (int k = 0; k < 8; k++)
+                if (k < NDIRS)
+                    v[u][k] = *reinterpret_cast<const uint4 *>(src + k * dir_stride);
         }
-        // key = S << 8 | d: the smallest key is the smallest S, then the smallest d
-        uint32_t key = 0xFFFFFFFFu;
 #pragma unroll
-        for (int j = 0; j < DPL; j++)
-            key = umin(key, (s[j] << 8) | (uint32_t)(l * DPL + j));
-        key = row_min16(key);
-        if (l == 0)
-            disp[p] = (uint8_t)(key & 0xFFu);
+        for (int u = 0; u < U; u++) {
+            uint32_t ev[4] = {0, 0, 0, 0}, od[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (k < NDIRS) {
+                    const uint32_t w[4] = {v[u][k].x, v[u][k].y, v[u][k].z, v[u][k].w};
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        ev[q] += w[q] & 0x00FF00FFu;
+                        od[q] += (w[q] >> 8) & 0x00FF00FFu;
+                    }
+                }
+            // key = S << 8 | d: the smallest key has the smallest S, then the smallest d
+            const uint32_t d0 = (uint32_t)sub * 16;
+            uint32_t key = 0xFFFFFFFFu;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t db = d0 + 4 * q;
+                key = umin(key, umin(((ev[q] & 0xFFFFu) << 8) | db, ((od[q] & 0xFFFFu) << 8) | (db + 1)));
+                key = umin(key, umin(((ev[q] >> 16) << 8) | (db + 2), ((od[q] >> 16) << 8) | (db + 3)));
+            }
+            key = group_min<LPP>(key);
+            if (sub == 0 && pix[u] < npix)
+                disp[pix[u]] = (uint8_t)(key & 0xFFu);
+        }
     }
 }
 
@@ -769,17 +864,31 @@ static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndi
     return -1;
 }
 
+template <int LPP>
+static void launch_wta_t(hipStream_t s, int blocks, const uint8_t *vol, size_t dir_stride, int ndirs,
+                         int n, uint8_t *disp)
+{
+    if (ndirs == 4)
+        hipLaunchKernelGGL((k_wta<LPP, 4>), dim3(blocks), dim3(256), 0, s, vol, dir_stride, n, disp);
+    else
+        hipLaunchKernelGGL((k_wta<LPP, 8>), dim3(blocks), dim3(256), 0, s, vol, dir_stride, n, disp);
+}
+
 static int launch_wta(hipStream_t s, const uint8_t *vol, size_t dir_stride, int ndirs, size_t npix,
                       int D, uint8_t *disp)
 {
-    const size_t groups = (npix + 15) / 16;
-    const int blocks = (int)std::min<size_t>(groups, 65536);
+    REQUIRE(ndirs >= 1 && ndirs <= 8 && npix < (1ull << 31), "sgmh: bad argmin shape");
+    const int ppw = 64 / (D / 16);                    // pixels per wave per batch
+    const size_t waves = (npix + (size_t)ppw * 2 - 1) / ((size_t)ppw * 2);
+    const int blocks = (int)std::min<size_t>((waves + 3) / 4, 8192);
+    const int n = (int)npix;
+    REQUIRE(ndirs == 4 || ndirs == 8, "sgmh: argmin over %d directions", ndirs);
     switch (D / 16) {
-    case 1: hipLaunchKernelGGL(k_wta<1>, dim3(blocks), dim3(256), 0, s, vol, dir_stride, ndirs, npix, D, disp); break;
-    case 2: hipLaunchKernelGGL(k_wta<2>, dim3(blocks), dim3(256), 0, s, vol, dir_stride, ndirs, npix, D, disp); break;
-    case 4: hipLaunchKernelGGL(k_wta<4>, dim3(blocks), dim3(256), 0, s, vol, dir_stride, ndirs, npix, D, disp); break;
-    case 8: hipLaunchKernelGGL(k_wta<8>, dim3(blocks), dim3(256), 0, s, vol, dir_stride, ndirs, npix, D, disp); break;
-    case 16: hipLaunchKernelGGL(k_wta<16>, dim3(blocks), dim3(256), 0, s, vol, dir_stride, ndirs, npix, D, disp); break;
+    case 1: launch_wta_t<1>(s, blocks, vol, dir_stride, ndirs, n, disp); break;
+    case 2: launch_wta_t<2>(s, blocks, vol, dir_stride, ndirs, n, disp); break;
+    case 4: launch_wta_t<4>(s, blocks, vol, dir_stride, ndirs, n, disp); break;
+    case 8: launch_wta_t<8>(s, blocks, vol, dir_stride, ndirs, n, disp); break;
+    case 16: launch_wta_t<16>(s, blocks, vol, dir_stride, ndirs, n, disp); break;
     default: set_err("sgmh: bad D %d", D); return -1;
     }
     CK(hipGetLastError());
