@@ -109,6 +109,11 @@ int sgmh_half_dev(sgmh_ctx *ctx, const uint8_t *d_img, int W, int H,
 int sgmh_path_plain_dev(sgmh_ctx *ctx, const uint64_t *d_cl, const uint64_t *d_cr,
                         int W, int H, int D, int dx, int dy, uint8_t *d_L, void *stream);
 
+/* Streaming copy of `bytes` (a multiple of 16) from src to dst on the device:
+ * the achievable-bandwidth ceiling and the known-size reference kernel for
+ * hardware byte counters (the reference's tools/gpu_bandwidth.cu:12-25). */
+int sgmh_copy_probe(sgmh_ctx *ctx, void *d_dst, const void *d_src, size_t bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -596,6 +596,15 @@ __device__ __forceinline__ void path_generic(const PathArgs &a, const DirTask &t
         block(s0, std::true_type{});
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by id, and each XCD has its
+// own L2.  Every direction's block range starts at a multiple of 8 and is
+// padded to one.  For columns, block i serves line group
+// (i % 8) * (n8 / 8) + i / 8: each XCD owns one contiguous slice of the
+// columns, so neighbouring columns -- whose 64-code disparity windows overlap
+// -- share an L2.  Rows gain nothing from it, and diagonals must stay
+// interleaved (their lengths vary along the range), so they keep block order.
+constexpr int kXcds = 8;
+
 template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM>
 __global__ __launch_bounds__(64) void k_path(PathArgs a)
 {
@@ -604,7 +613,12 @@ __global__ __launch_bounds__(64) void k_path(PathArgs a)
     while (k + 1 < a.ndirs && (int)blockIdx.x >= a.dir[k + 1].first_block)
         k++;
     const DirTask t = a.dir[k];
-    const int base = ((int)blockIdx.x - t.first_block) * kLinesPerWave;
+    const int i = (int)blockIdx.x - t.first_block;
+    const int n8 = (t.nlines + kLinesPerWave * kXcds - 1) / (kLinesPerWave * kXcds) * kXcds;
+    const int grp = t.dx == 0 ? (i % kXcds) * (n8 / kXcds) + i / kXcds : i;
+    const int base = grp * kLinesPerWave;
+    if (base >= t.nlines)
+        return; // padding block
     int maxsteps = 0;
 #pragma unroll
     for (int q = 0; q < kLinesPerWave; q++)
@@ -700,6 +714,19 @@ __global__ __launch_bounds__(256) void k_wta(const uint8_t *__restrict__ L, size
                 disp[pix[u]] = (uint8_t)(key & 0xFFu);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// streaming-copy probe: the achievable HBM ceiling and the reference size for
+// scaling hardware byte counters (the pattern of the reference's
+// tools/gpu_bandwidth.cu:12-25: a float4 copy, one element per thread)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -852,7 +879,9 @@ static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndi
         t.nlines = t.dy == 0 ? a.H : (t.dx == 0 ? a.W : a.W + a.H - 1);
         t.first_block = blocks;
         t.L = vol + (size_t)k * dir_stride;
-        blocks += (t.nlines + kLinesPerWave - 1) / kLinesPerWave;
+        // line groups of this direction, padded to a multiple of the XCD count
+        const int groups = (t.nlines + kLinesPerWave - 1) / kLinesPerWave;
+        blocks += (groups + kXcds - 1) / kXcds * kXcds;
     }
     if (from_census && !sat && !decimate)
         return launch_path_t<true, false, false>(s, a, blocks);
@@ -1261,6 +1290,18 @@ int sgmh_path_plain_dev(sgmh_ctx *c, const uint64_t *cl, const uint64_t *cr, int
     CK(hipMemcpyAsync(pr + kLead, cr, N * 8, hipMemcpyDeviceToDevice, s));
     const int dirs[1][3] = {{dx, dy, (int)kPlainP1}};
     return plain_paths(c, s, pl, pr, W, H, D, dirs, 1, L, N * D);
+}
+
+int sgmh_copy_probe(sgmh_ctx *c, void *dst, const void *src, size_t bytes, void *stream)
+{
+    REQUIRE(c && dst && src && bytes % 16 == 0, "sgmh_copy_probe: bad arguments");
+    CK(hipSetDevice(c->device));
+    hipStream_t s = pick(c, stream);
+    const size_t n = bytes / 16;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 1u << 20);
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, n);
+    CK(hipGetLastError());
+    return 0;
 }
 
 } // extern "C"

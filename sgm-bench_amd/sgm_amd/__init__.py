@@ -94,9 +94,10 @@ def hip_lib() -> ctypes.CDLL:
         h.sgmh_sobel_dev.argtypes = [_P, _P, _I, _I, _P, _P]
         h.sgmh_half_dev.argtypes = [_P, _P, _I, _I, _P, _P]
         h.sgmh_path_plain_dev.argtypes = [_P, _P, _P, _I, _I, _I, _I, _I, _P, _P]
+        h.sgmh_copy_probe.argtypes = [_P, _P, _P, ctypes.c_size_t, _P]
         for name in ("sgmh_ctx_create", "sgmh_set_timing", "sgmh_get_times", "sgmh_plain",
                      "sgmh_plain_dev", "sgmh_ms", "sgmh_ms_dev", "sgmh_census_dev",
-                     "sgmh_sobel_dev", "sgmh_half_dev", "sgmh_path_plain_dev"):
+                     "sgmh_sobel_dev", "sgmh_half_dev", "sgmh_path_plain_dev", "sgmh_copy_probe"):
             getattr(h, name).restype = _I
         _hip = h
     return _hip
@@ -266,6 +267,9 @@ class Context:
 
     def half_dev(self, d_img: int, W: int, H: int, d_out: int, stream: int = 0) -> None:
         self._ck(self._lib.sgmh_half_dev(self._h, d_img, W, H, d_out, stream or None))
+
+    def copy_probe(self, d_dst: int, d_src: int, nbytes: int, stream: int = 0) -> None:
+        self._ck(self._lib.sgmh_copy_probe(self._h, d_dst, d_src, nbytes, stream or None))
 
     def path_plain_dev(self, d_cl: int, d_cr: int, W: int, H: int, D: int, dx: int, dy: int,
                        d_L: int, stream: int = 0) -> None:
