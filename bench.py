@@ -8,6 +8,9 @@ A step is one frame of the named workload, inputs already resident in HBM:
   P  plain SGM,  640x480,  D=64, 4 paths
   B  multiscale 8-path two-scale SGM, 1920x800,  D=64
   C  multiscale 8-path two-scale SGM, 1920x1080, D=64
+  batch  24 pairs of uneven size in one submission (8x640x480 D64, 8x1280x720
+     D64, 4x1920x800 D128 multiscale, 4x1920x1080 D128), one graph launch per
+     step; frames/s against a loop of single-pair launches of the same pairs
 The last frame of the timed loop is hash-checked against the compiled
 reference oracle's golden for that scene; a mismatch exits 2, as the
 reference harness does (common/harness.c:236).
@@ -37,7 +40,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="A", choices=["A", "P", "B", "C"])
+    ap.add_argument("--workload", default="A", choices=["A", "P", "B", "C", "batch"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget per CPU-baseline leg")
@@ -110,8 +113,107 @@ def cpu_baseline(L, R, mode, D, paths, budget_s, mdes_per_frame):
     }
 
 
+def bench_batch(a):
+    import numpy as np
+    import torch
+    import sgm_amd
+
+    specs = sgm_amd.batch_spec()
+    with open(os.path.join(ROOT, "tests", "golden", "goldens.json")) as f:
+        gold = {g["name"]: g["fnv"] for g in json.load(f)["batch"]}
+    keep, pairs, scenes = [], [], []
+    for sp in specs:
+        L, R = sgm_amd.make_scene(sp["scene"])
+        scenes.append((L, R))
+        dl, dr = torch.from_numpy(L).cuda(), torch.from_numpy(R).cuda()
+        oh, ow = (sp["H"], sp["W"]) if sp["mode"] == sgm_amd.PLAIN else (sp["H"] // 2, sp["W"] // 2)
+        do = torch.empty((oh, ow), dtype=torch.uint8, device="cuda")
+        keep.append((dl, dr, do))
+        pairs.append(dict(d_left=dl.data_ptr(), d_right=dr.data_ptr(), d_disp=do.data_ptr(), W=sp["W"],
+                          H=sp["H"], D=sp["D"], mode=sp["mode"], paths=sp["paths"]))
+    ctx = sgm_amd.Context(0)
+    b = ctx.batch(pairs, sgm_amd.BATCH_AUTOTUNE)
+    sp_ = torch.cuda.current_stream().cuda_stream
+
+    def timed(fn, steps, warm):
+        for _ in range(warm):
+            fn(sp_)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn(sp_)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def check():
+        bad = [s["name"] for s, (_, _, do) in zip(specs, keep)
+               if sgm_amd.fnv1a64(do.cpu().numpy()) != gold[s["name"]]]
+        return bad
+
+    t_loop = timed(b.run_loop, a.steps, a.warmup)
+    bad_loop = check()
+    for _, _, do in keep:
+        do.fill_(0)
+    t_graph = timed(b.run, a.steps, a.warmup)
+    bad = check()
+    n = len(specs)
+    fps = n * a.steps / t_graph
+    fps_loop = n * a.steps / t_loop
+    mde = sum((s["W"] * s["H"] if s["mode"] == sgm_amd.PLAIN else s["W"] * s["H"] * 5 // 4) * s["D"]
+              for s in specs) / 1e6
+    rec = {
+        "metric": "batched frames/s, 24 uneven pairs, bit-exact",
+        "value": round(fps, 1),
+        "unit": "frames/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(1000 * t_graph / a.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8/u16 integer",
+        "data": "synthetic (24 seeded planes scenes, generated on the host)",
+        "config": {"workload": "batch: 8x640x480 D64 + 8x1280x720 D64 + 4x1920x800 D128 multiscale "
+                               "+ 4x1920x1080 D128, one hipGraph launch per step"},
+        "loop_frames_per_s": round(fps_loop, 1),
+        "speedup_vs_loop": round(fps / fps_loop, 3),
+        "MDE_per_s": round(mde * a.steps / t_graph, 1),
+        "golden_match": not bad and not bad_loop,
+        "golden_failures": bad + ["loop:" + x for x in bad_loop],
+        "plan": b.describe(),
+    }
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        # one pair of each kind through the OpenMP oracle, weighted by its count
+        kinds, total = {}, 0.0
+        for s, (L, R) in zip(specs, scenes):
+            key = (s["W"], s["H"], s["D"], s["mode"])
+            if key not in kinds:
+                t0 = time.perf_counter()
+                if s["mode"] == sgm_amd.PLAIN:
+                    oracle.sgm_plain(L, R, s["D"], s["paths"])
+                else:
+                    oracle.sgm_ms(L, R, s["D"])
+                kinds[key] = time.perf_counter() - t0
+            total += kinds[key]
+        rec["cpu_baseline"] = {"value": round(n / total, 3), "unit": "frames/s",
+                               "cores": oracle.omp_threads(), "kind": "port",
+                               "sample": "one pair of each of the 4 kinds through the oracle's OpenMP "
+                                         "restatement, weighted by its count in the batch (%.1f s)"
+                                         % sum(kinds.values())}
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+    b.close()
+    ctx.close()
+    return 0 if rec["golden_match"] else 2
+
+
 def main():
     a = parse()
+    if a.workload == "batch":
+        if a.gpus != 1:
+            print("bench.py: one GPU only", file=sys.stderr)
+            return 1
+        return bench_batch(a)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not supported "
               "(batched pairs would shard across GPUs by pair, see DESIGN.md).", file=sys.stderr)

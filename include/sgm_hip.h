@@ -109,6 +109,42 @@ int sgmh_half_dev(sgmh_ctx *ctx, const uint8_t *d_img, int W, int H,
 int sgmh_path_plain_dev(sgmh_ctx *ctx, const uint64_t *d_cl, const uint64_t *d_cr,
                         int W, int H, int D, int dx, int dy, uint8_t *d_L, void *stream);
 
+/* ---- batched mode: many pairs of uneven size, D and configuration in one
+ *      submission (no reference counterpart; the reference runs one pair per
+ *      binary invocation, common/harness.c:150-158) ---- */
+
+#define SGMH_PLAIN 0      /* plain path, sgmh_plain semantics */
+#define SGMH_MULTISCALE 1 /* multiscale path, sgmh_ms semantics */
+
+typedef struct {
+    const uint8_t *d_left, *d_right; /* device, W*H each */
+    int W, H, D;
+    int mode;                        /* SGMH_PLAIN or SGMH_MULTISCALE */
+    int paths;                       /* plain: 4 or 8; multiscale: 8 */
+    uint8_t *d_disp;                 /* device; W*H (plain) or (W/2)*(H/2) */
+} sgmh_pair;
+
+typedef struct sgmh_batch sgmh_batch;
+
+#define SGMH_BATCH_AUTOTUNE 1 /* time candidate schedules on this batch, keep the best */
+#define SGMH_BATCH_NO_GRAPH 2 /* launch the phase kernels directly, no hipGraph */
+
+/* Plan a batch: allocate every pair's workspace, build the work lists of the
+ * persistent per-phase kernels (pre-filter / census, cost, path aggregation,
+ * sum + argmin), optionally autotune the schedule, and capture the phases as
+ * one hipGraph.  The device buffers named in `pairs` are bound into the plan:
+ * later runs read whatever they then hold. */
+int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgmh_batch **out);
+/* Run the planned batch on `stream` (NULL: the HIP default stream), asynchronous. */
+int sgmh_batch_run(sgmh_batch *b, void *stream);
+/* The comparison baseline: the same pairs, one sgmh_plain_dev / sgmh_ms_dev
+ * call after another, on `stream`. */
+int sgmh_batch_run_loop(sgmh_batch *b, void *stream);
+/* JSON description of the plan: pairs, items per phase, chosen schedule and
+ * the autotuner's timings.  Returns the length written (truncated to len-1). */
+int sgmh_batch_describe(const sgmh_batch *b, char *buf, size_t len);
+void sgmh_batch_destroy(sgmh_batch *b);
+
 /* Streaming copy of `bytes` (a multiple of 16) from src to dst on the device:
  * the achievable-bandwidth ceiling and the known-size reference kernel for
  * hardware byte counters (the reference's tools/gpu_bandwidth.cu:12-25). */

@@ -1,0 +1,784 @@
+// sgm_batch.hip -- batched mode: many stereo pairs of uneven size, D and
+// configuration in one submission.
+//
+// A plan turns the pairs into work lists for four persistent phase kernels,
+// each of which serves every pair at once:
+//   k_batch_pre   census tiles (plain), and for multiscale pairs the half
+//                 images, Sobel-X images and census tiles of both scales
+//                 (three launches, the dependency levels of the pre-filters)
+//   k_batch_cost  multiscale cost volumes (two launches: half scale, then the
+//                 full scale merged with it)
+//   k_batch_path  path aggregation: one item is one group of 4 scanlines of
+//                 one direction of one pair; waves pull items from a global
+//                 counter, longest lines first by default
+//   k_batch_wta   sum over directions + argmin, all pairs and both
+//                 multiscale stages
+// The launches are captured once into a hipGraph; a run is one graph launch.
+// The autotuner times candidate schedules (persistent waves per SIMD for the
+// path phase, item order) on the batch itself and keeps the fastest.
+//
+// Every pair is computed by the same device functions as the single-pair
+// entry points (sgm_device.h), so each stays bit-exact to its oracle.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "sgm_device.h"
+#include "sgm_internal.h"
+
+using namespace sgmk;
+
+namespace {
+
+enum PreKind { PRE_HALF = 0, PRE_SOBEL = 1, PRE_CENSUS = 2 };
+
+struct PreTask {
+    const uint8_t *src;
+    void *dst;       // uint8_t image (half, Sobel) or uint64_t codes (census)
+    int W, H;        // source image size
+    int kind;
+    int ostride;     // census row stride in codes
+    int tiles_x;     // tiles per row
+    int first_item;  // first item of this task in the launch
+};
+
+struct CostTask {
+    const uint64_t *cl, *cr;
+    const uint8_t *merge; // nullptr: half scale; else the half-scale volume
+    uint8_t *C;
+    int W, H, D;
+    uint32_t invalid;
+    int first_item;
+};
+
+// Field names match PathArgs so the path device functions accept it.
+struct BatchPathTask {
+    const uint64_t *cl, *cr;
+    const uint8_t *cost;
+    int W, H, D;
+    uint32_t P2, invalid;
+    uint32_t out_bytes;
+    DirTask dir;
+    int variant; // kVar* below
+};
+
+enum {
+    kVarPlain32 = 0, kVarPlain64, kVarPlain128,
+    kVarMs32, kVarMs64, kVarMs128,          // multiscale stage 1 (full-layout store)
+    kVarMsDec32, kVarMsDec64, kVarMsDec128, // multiscale stage 2 (decimated store)
+};
+
+struct WtaTask {
+    const uint8_t *L;
+    size_t dir_stride;
+    int ndirs, npix, D;
+    uint8_t *disp;
+    int first_item;
+};
+
+// The items of a launch are split into contiguous chunks, one per block (or
+// wave); a chunk crosses few task boundaries, so the task is found once and
+// then advanced as the item index passes the next task's first item.
+template <class T>
+__device__ __forceinline__ int find_task(const T *__restrict__ tasks, int ntasks, int item)
+{
+    int lo = 0, hi = ntasks - 1; // last task whose first_item <= item
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tasks[mid].first_item <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+template <class T>
+struct TaskCursor {
+    const T *tasks;
+    int ntasks, t, next; // current task, first item of the one after it
+    T cur;
+    __device__ void seek(int item)
+    {
+        t = find_task(tasks, ntasks, item);
+        cur = tasks[t];
+        next = t + 1 < ntasks ? tasks[t + 1].first_item : 0x7FFFFFFF;
+    }
+    __device__ void advance(int item)
+    {
+        while (item >= next) {
+            t++;
+            cur = tasks[t];
+            next = t + 1 < ntasks ? tasks[t + 1].first_item : 0x7FFFFFFF;
+        }
+    }
+};
+
+__device__ __forceinline__ void chunk_of(int unit, int nunits, int nitems, int &begin, int &end)
+{
+    const int per = (nitems + nunits - 1) / nunits;
+    begin = min(nitems, unit * per);
+    end = min(nitems, begin + per);
+}
+
+__global__ __launch_bounds__(64) void k_batch_reset(int *counter) { if (threadIdx.x == 0) *counter = 0; }
+
+__global__ __launch_bounds__(256) void k_batch_pre(const PreTask *__restrict__ tasks, int ntasks, int nitems)
+{
+    __shared__ uint32_t tile[kCenRows * kCenRowWords];
+    int begin, end;
+    chunk_of(blockIdx.x, gridDim.x, nitems, begin, end);
+    if (begin >= end)
+        return;
+    TaskCursor<PreTask> c{tasks, ntasks};
+    c.seek(begin);
+    for (int item = begin; item < end; item++) {
+        c.advance(item);
+        const PreTask &tk = c.cur;
+        const int local = item - tk.first_item;
+        const int tx = local % tk.tiles_x, ty = local / tk.tiles_x;
+        if (tk.kind == PRE_CENSUS) {
+            census_tile(tk.src, (uint64_t *)tk.dst, tk.W, tk.H, tk.ostride, tx * kCenTX, ty * kCenTY, tile);
+        } else {
+            const int x = tx * 64 + (threadIdx.x & 63), y = ty * 4 + (threadIdx.x >> 6);
+            if (tk.kind == PRE_HALF) {
+                if (x < tk.W / 2 && y < tk.H / 2)
+                    half_px(tk.src, tk.W, (uint8_t *)tk.dst, tk.W / 2, x, y);
+            } else if (x < tk.W && y < tk.H) {
+                sobel_px(tk.src, tk.W, tk.H, (uint8_t *)tk.dst, x, y);
+            }
+        }
+    }
+}
+
+// one item = 16 pixels: 4 waves x 4 sixteen-lane rows
+__global__ __launch_bounds__(256) void k_batch_cost(const CostTask *__restrict__ tasks, int ntasks, int nitems)
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+    int begin, end;
+    chunk_of(blockIdx.x, gridDim.x, nitems, begin, end);
+    if (begin >= end)
+        return;
+    TaskCursor<CostTask> c{tasks, ntasks};
+    c.seek(begin);
+    for (int item = begin; item < end; item++) {
+        c.advance(item);
+        const CostTask &tk = c.cur;
+        const size_t p = (size_t)(item - tk.first_item) * 16 + (threadIdx.x >> 6) * 4 + g;
+        if (p >= (size_t)tk.W * tk.H)
+            continue;
+        switch (tk.D) {
+        case 32: cost_px<2>(tk.cl, tk.cr, tk.W, tk.D, tk.invalid, tk.merge, tk.C, p, l); break;
+        case 64: cost_px<4>(tk.cl, tk.cr, tk.W, tk.D, tk.invalid, tk.merge, tk.C, p, l); break;
+        default: cost_px<8>(tk.cl, tk.cr, tk.W, tk.D, tk.invalid, tk.merge, tk.C, p, l); break;
+        }
+    }
+}
+
+// One wave per block; items are (task << 16 | line group) in queue order.
+__global__ __launch_bounds__(64) void k_batch_path(const BatchPathTask *__restrict__ tasks,
+                                                   const uint32_t *__restrict__ items, int nitems,
+                                                   int *__restrict__ counter)
+{
+    for (;;) {
+        int it = 0;
+        if (threadIdx.x == 0)
+            it = atomicAdd(counter, 1);
+        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+        if (it >= nitems)
+            break;
+        const uint32_t code = items[it];
+        const BatchPathTask tk = tasks[code >> 16];
+        const int grp = (int)(code & 0xFFFFu);
+        switch (tk.variant) {
+        case kVarPlain32: path_group<2, true, false, false>(tk, tk.dir, grp); break;
+        case kVarPlain64: path_group<4, true, false, false>(tk, tk.dir, grp); break;
+        case kVarPlain128: path_group<8, true, false, false>(tk, tk.dir, grp); break;
+        case kVarMs32: path_group<2, false, true, false>(tk, tk.dir, grp); break;
+        case kVarMs64: path_group<4, false, true, false>(tk, tk.dir, grp); break;
+        case kVarMs128: path_group<8, false, true, false>(tk, tk.dir, grp); break;
+        case kVarMsDec32: path_group<2, false, true, true>(tk, tk.dir, grp); break;
+        case kVarMsDec64: path_group<4, false, true, true>(tk, tk.dir, grp); break;
+        default: path_group<8, false, true, true>(tk, tk.dir, grp); break;
+        }
+    }
+}
+
+// one item = one wta_batch of one task (64/LPP * kWtaBatch pixels); each wave
+// takes a contiguous chunk of items
+__global__ __launch_bounds__(256) void k_batch_wta(const WtaTask *__restrict__ tasks, int ntasks, int nitems)
+{
+    const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    int begin, end;
+    chunk_of(wave, (int)gridDim.x * 4, nitems, begin, end);
+    if (begin >= end)
+        return;
+    TaskCursor<WtaTask> c{tasks, ntasks};
+    c.seek(begin);
+    for (int item = begin; item < end; item++) {
+        c.advance(item);
+        const WtaTask &tk = c.cur;
+        const int lpp = tk.D / 16;
+        const int b0 = (item - tk.first_item) * (64 / lpp) * kWtaBatch;
+        const bool eight = tk.ndirs == 8;
+        switch (lpp) {
+        case 2: eight ? wta_batch<2, 8>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0)
+                      : wta_batch<2, 4>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0); break;
+        case 4: eight ? wta_batch<4, 8>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0)
+                      : wta_batch<4, 4>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0); break;
+        default: eight ? wta_batch<8, 8>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0)
+                       : wta_batch<8, 4>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0); break;
+        }
+    }
+}
+
+template <class T>
+struct DevArray {
+    std::vector<T> host;
+    T *dev = nullptr;
+};
+
+struct Schedule {
+    int path_waves_per_simd = 2;
+    int order = 0; // 0: longest lines first; 1: pair by pair, directions in order
+};
+
+} // namespace
+
+// The pairs of one class (plain or multiscale) and their phase work lists.
+struct BatchLane {
+    DevArray<PreTask> pre[3];
+    DevArray<CostTask> cost[2];
+    DevArray<BatchPathTask> path;
+    std::vector<uint32_t> items_lpt, items_pair;
+    uint32_t *d_items = nullptr; // both orders, back to back
+    DevArray<WtaTask> wta;
+    int pre_n[3] = {0, 0, 0}, cost_n[2] = {0, 0}, wta_n = 0; // items per launch
+    int *d_counter = nullptr;
+    bool empty() const { return path.host.empty(); }
+};
+
+struct sgmh_batch {
+    sgmh_ctx *ctx = nullptr;
+    std::vector<sgmh_pair> pairs;
+    void *arena = nullptr;
+    size_t arena_bytes = 0;
+    BatchLane lane[2]; // 0: plain pairs, 1: multiscale pairs
+    int cus = 256;
+    Schedule sched;
+    std::string tuning; // JSON fragment
+    bool graph_ok = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr, cap2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+namespace {
+
+template <class T>
+int upload(DevArray<T> &a)
+{
+    if (a.host.empty())
+        return 0;
+    CK(hipMalloc(&a.dev, sizeof(T) * a.host.size()));
+    CK(hipMemcpy(a.dev, a.host.data(), sizeof(T) * a.host.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int plain_variant(int D) { return D == 32 ? kVarPlain32 : D == 64 ? kVarPlain64 : kVarPlain128; }
+int ms_variant(int D, bool dec)
+{
+    const int base = D == 32 ? 0 : D == 64 ? 1 : 2;
+    return (dec ? kVarMsDec32 : kVarMs32) + base;
+}
+
+// total pixel-steps of a line group, the LPT key
+long group_work(const BatchPathTask &t, int grp)
+{
+    const int W = t.W, H = t.H, dx = t.dir.dx, dy = t.dir.dy;
+    if (dy == 0)
+        return (long)W * t.D;
+    if (dx == 0)
+        return (long)H * t.D;
+    long best = 0;
+    for (int q = 0; q < kLinesPerWave; q++) {
+        const int i = grp * kLinesPerWave + q;
+        if (i >= t.dir.nlines)
+            break;
+        int x, y;
+        if (i < W) {
+            x = dx > 0 ? i : W - 1 - i;
+            y = dy > 0 ? 0 : H - 1;
+        } else {
+            const int j = i - W + 1;
+            x = dx > 0 ? 0 : W - 1;
+            y = dy > 0 ? j : H - 1 - j;
+        }
+        const int sx = dx > 0 ? W - x : x + 1, sy = dy > 0 ? H - y : y + 1;
+        best = std::max<long>(best, std::min(sx, sy));
+    }
+    return best * t.D;
+}
+
+int enqueue_lane(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc)
+{
+    const int pre_blocks = b->cus * 8;
+    for (int k = 0; k < 3; k++)
+        if (ln.pre_n[k])
+            hipLaunchKernelGGL(k_batch_pre, dim3(std::min(ln.pre_n[k], pre_blocks)), dim3(256), 0, s,
+                               ln.pre[k].dev, (int)ln.pre[k].host.size(), ln.pre_n[k]);
+    for (int k = 0; k < 2; k++)
+        if (ln.cost_n[k])
+            hipLaunchKernelGGL(k_batch_cost, dim3(std::min(ln.cost_n[k], pre_blocks)), dim3(256), 0, s,
+                               ln.cost[k].dev, (int)ln.cost[k].host.size(), ln.cost_n[k]);
+    CK(hipGetLastError());
+    if (!ln.path.host.empty()) {
+        const int n = (int)ln.items_lpt.size();
+        const uint32_t *items = ln.d_items + (sc.order == 0 ? 0 : n);
+        const int waves = std::min(n, b->cus * 4 * sc.path_waves_per_simd);
+        hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, ln.d_counter);
+        hipLaunchKernelGGL(k_batch_path, dim3(waves), dim3(64), 0, s, ln.path.dev, items, n, ln.d_counter);
+    }
+    if (ln.wta_n)
+        hipLaunchKernelGGL(k_batch_wta, dim3(std::min((ln.wta_n + 3) / 4, b->cus * 8)), dim3(256), 0, s,
+                           ln.wta.dev, (int)ln.wta.host.size(), ln.wta_n);
+    CK(hipGetLastError());
+    return 0;
+}
+
+// Both lanes, the multiscale one on a forked stream: they share no data, so
+// one lane's memory-bound phases overlap the other's path aggregation.
+int enqueue_phases(sgmh_batch *b, hipStream_t s, const Schedule &sc)
+{
+    const bool two = !b->lane[0].empty() && !b->lane[1].empty();
+    if (!two)
+        return enqueue_lane(b, b->lane[b->lane[0].empty() ? 1 : 0], s, sc);
+    CK(hipEventRecord(b->fork, s));
+    CK(hipStreamWaitEvent(b->cap2, b->fork, 0));
+    if (enqueue_lane(b, b->lane[1], b->cap2, sc) || enqueue_lane(b, b->lane[0], s, sc))
+        return -1;
+    CK(hipEventRecord(b->join, b->cap2));
+    CK(hipStreamWaitEvent(s, b->join, 0));
+    return 0;
+}
+
+int capture(sgmh_batch *b)
+{
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    CK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_phases(b, b->cap, b->sched);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (rc)
+        return -1;
+    CK(e);
+    b->graph = g;
+    CK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_ok = true;
+    return 0;
+}
+
+// Time each candidate schedule (median of 3 runs after one warm-up) and keep
+// the fastest.
+int autotune(sgmh_batch *b)
+{
+    std::vector<Schedule> cands;
+    for (int order = 0; order < 2; order++)
+        for (int pw = 1; pw <= 4; pw++) {
+            Schedule sc;
+            sc.order = order;
+            sc.path_waves_per_simd = pw;
+            cands.push_back(sc);
+        }
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    hipStream_t s = b->cap;
+    double best = 1e30;
+    std::string js = "[";
+    for (size_t c = 0; c < cands.size(); c++) {
+        float t[3];
+        if (enqueue_phases(b, s, cands[c]))
+            return -1;
+        for (int r = 0; r < 3; r++) {
+            CK(hipEventRecord(e0, s));
+            if (enqueue_phases(b, s, cands[c]))
+                return -1;
+            CK(hipEventRecord(e1, s));
+            CK(hipEventSynchronize(e1));
+            CK(hipEventElapsedTime(&t[r], e0, e1));
+        }
+        std::sort(t, t + 3);
+        char buf[160];
+        snprintf(buf, sizeof buf, "%s{\"order\":\"%s\",\"path_waves_per_simd\":%d,\"ms\":%.4f}",
+                 c ? "," : "", cands[c].order == 0 ? "longest-first" : "pair-major",
+                 cands[c].path_waves_per_simd, t[1]);
+        js += buf;
+        if (t[1] < best) {
+            best = t[1];
+            b->sched = cands[c];
+        }
+    }
+    js += "]";
+    b->tuning = js;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgmh_batch **out)
+{
+    REQUIRE(ctx && pairs && out && n > 0, "sgmh_batch_plan: bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        const sgmh_pair &p = pairs[i];
+        REQUIRE(p.d_left && p.d_right && p.d_disp, "sgmh_batch_plan: pair %d has a NULL buffer", i);
+        if (check_dims(p.W, p.H, p.D))
+            return -1;
+        REQUIRE(p.D == 32 || p.D == 64 || p.D == 128,
+                "sgmh_batch_plan: pair %d D=%d (batched mode runs D = 32, 64, 128)", i, p.D);
+        REQUIRE(p.mode == SGMH_PLAIN || p.mode == SGMH_MULTISCALE, "sgmh_batch_plan: pair %d mode", i);
+        if (p.mode == SGMH_PLAIN)
+            REQUIRE(p.paths == 4 || p.paths == 8, "sgmh_batch_plan: pair %d paths=%d", i, p.paths);
+        else
+            REQUIRE(p.W % 2 == 0 && p.H % 2 == 0 && p.W >= 32 && p.H >= 32,
+                    "sgmh_batch_plan: pair %d multiscale needs even W, H >= 32", i);
+        const size_t N = (size_t)p.W * p.H;
+        REQUIRE(N * p.D < (1ull << 31) && (N + kLead) * 8 < (1ull << 31),
+                "sgmh_batch_plan: pair %d exceeds the 2 GiB per-volume limit", i);
+    }
+    CK(hipSetDevice(ctx->device));
+    sgmh_batch *b = new sgmh_batch();
+    b->ctx = ctx;
+    b->pairs.assign(pairs, pairs + n);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+        b->cus = prop.multiProcessorCount;
+
+    // ---- workspace layout ----
+    struct Ws {
+        size_t cl, cr, vol;                                  // plain
+        size_t hl, hr, sl, sr, shl, shr, cl1, cr1, cl2, cr2; // multiscale
+        size_t c1, c2, v1, v2, d1;
+    };
+    std::vector<Ws> ws(n);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += align256(bytes);
+        return o;
+    };
+    for (int i = 0; i < n; i++) {
+        const sgmh_pair &p = pairs[i];
+        const size_t N = (size_t)p.W * p.H, Nh = N / 4, D = (size_t)p.D;
+        Ws &w = ws[i];
+        if (p.mode == SGMH_PLAIN) {
+            w.cl = take((N + kLead) * 8);
+            w.cr = take((N + kLead) * 8);
+            w.vol = take(N * D * p.paths);
+        } else {
+            w.hl = take(Nh);
+            w.hr = take(Nh);
+            w.sl = take(N);
+            w.sr = take(N);
+            w.shl = take(Nh);
+            w.shr = take(Nh);
+            w.cl1 = take(Nh * 8);
+            w.cr1 = take(Nh * 8);
+            w.cl2 = take(N * 8);
+            w.cr2 = take(N * 8);
+            w.c1 = take(Nh * D);
+            w.c2 = take(N * D);
+            w.v1 = take(Nh * D * 8);
+            w.v2 = take(Nh * D * 8);
+            w.d1 = take(Nh);
+        }
+    }
+    b->arena_bytes = off;
+    if (hipMalloc(&b->arena, off) != hipSuccess) {
+        set_err("sgmh_batch_plan: cannot allocate %zu bytes of workspace", off);
+        delete b;
+        return -1;
+    }
+    uint8_t *A = (uint8_t *)b->arena;
+
+    // ---- phase work lists ----
+    int pre_items[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    int li = 0; // lane of the pair being added
+    auto add_pre = [&](int level, int kind, const uint8_t *src, void *dst, int W, int H, int ostride) {
+        PreTask t;
+        t.src = src;
+        t.dst = dst;
+        t.W = W;
+        t.H = H;
+        t.kind = kind;
+        t.ostride = ostride;
+        int ty;
+        if (kind == PRE_CENSUS) {
+            t.tiles_x = (W + kCenTX - 1) / kCenTX;
+            ty = (H + kCenTY - 1) / kCenTY;
+        } else {
+            const int ow = kind == PRE_HALF ? W / 2 : W, oh = kind == PRE_HALF ? H / 2 : H;
+            t.tiles_x = (ow + 63) / 64;
+            ty = (oh + 3) / 4;
+        }
+        t.first_item = pre_items[li][level];
+        pre_items[li][level] += t.tiles_x * ty;
+        b->lane[li].pre[level].host.push_back(t);
+    };
+    int cost_items[2][2] = {{0, 0}, {0, 0}};
+    auto add_cost = [&](int level, const uint64_t *cl, const uint64_t *cr, const uint8_t *merge, uint8_t *C,
+                        int W, int H, int D) {
+        CostTask t{cl, cr, merge, C, W, H, D, kMsInvalid, cost_items[li][level]};
+        cost_items[li][level] += (int)(((size_t)W * H + 15) / 16);
+        b->lane[li].cost[level].host.push_back(t);
+    };
+    int wta_items[2] = {0, 0};
+    auto add_wta = [&](const uint8_t *L, size_t stride, int ndirs, int npix, int D, uint8_t *disp) {
+        WtaTask t{L, stride, ndirs, npix, D, disp, wta_items[li]};
+        const int per = (64 / (D / 16)) * kWtaBatch;
+        wta_items[li] += (npix + per - 1) / per;
+        b->lane[li].wta.host.push_back(t);
+    };
+    auto add_path = [&](const uint64_t *cl, const uint64_t *cr, const uint8_t *cost, int W, int H, int D,
+                        uint32_t P2, uint32_t inval, uint32_t out_bytes, int dx, int dy, int P1,
+                        uint8_t *L, int variant) {
+        BatchPathTask t;
+        t.cl = cl;
+        t.cr = cr;
+        t.cost = cost;
+        t.W = W;
+        t.H = H;
+        t.D = D;
+        t.P2 = P2;
+        t.invalid = inval;
+        t.out_bytes = out_bytes;
+        t.dir.dx = dx;
+        t.dir.dy = dy;
+        t.dir.P1 = P1;
+        t.dir.nlines = dy == 0 ? H : (dx == 0 ? W : W + H - 1);
+        t.dir.first_block = 0;
+        t.dir.L = L;
+        t.variant = variant;
+        b->lane[li].path.host.push_back(t);
+    };
+    for (int i = 0; i < n; i++) {
+        const sgmh_pair &p = pairs[i];
+        const Ws &w = ws[i];
+        const int W = p.W, H = p.H, D = p.D, hw = W / 2, hh = H / 2;
+        const size_t N = (size_t)W * H, Nh = (size_t)hw * hh;
+        li = p.mode == SGMH_PLAIN ? 0 : 1;
+        if (p.mode == SGMH_PLAIN) {
+            uint64_t *cl = (uint64_t *)(A + w.cl), *cr = (uint64_t *)(A + w.cr);
+            add_pre(0, PRE_CENSUS, p.d_left, cl + kLead, W, H, W);
+            add_pre(0, PRE_CENSUS, p.d_right, cr + kLead, W, H, W);
+            for (int k = 0; k < p.paths; k++)
+                add_path(cl, cr, nullptr, W, H, D, kPlainP2, kPlainInvalid, (uint32_t)(N * D),
+                         kDirsPlain[k][0], kDirsPlain[k][1], (int)kPlainP1, A + w.vol + (size_t)k * N * D,
+                         plain_variant(D));
+            add_wta(A + w.vol, N * D, p.paths, (int)N, D, p.d_disp);
+        } else {
+            uint8_t *hl = A + w.hl, *hr = A + w.hr, *sl = A + w.sl, *sr = A + w.sr;
+            uint8_t *shl = A + w.shl, *shr = A + w.shr;
+            uint64_t *cl1 = (uint64_t *)(A + w.cl1), *cr1 = (uint64_t *)(A + w.cr1);
+            uint64_t *cl2 = (uint64_t *)(A + w.cl2), *cr2 = (uint64_t *)(A + w.cr2);
+            uint8_t *C1 = A + w.c1, *C2 = A + w.c2, *V1 = A + w.v1, *V2 = A + w.v2;
+            add_pre(0, PRE_HALF, p.d_left, hl, W, H, 0);
+            add_pre(0, PRE_HALF, p.d_right, hr, W, H, 0);
+            add_pre(0, PRE_SOBEL, p.d_left, sl, W, H, 0);
+            add_pre(0, PRE_SOBEL, p.d_right, sr, W, H, 0);
+            add_pre(1, PRE_SOBEL, hl, shl, hw, hh, 0);
+            add_pre(1, PRE_SOBEL, hr, shr, hw, hh, 0);
+            add_pre(1, PRE_CENSUS, sl, cl2, W, H, W);
+            add_pre(1, PRE_CENSUS, sr, cr2, W, H, W);
+            add_pre(2, PRE_CENSUS, shl, cl1, hw, hh, hw);
+            add_pre(2, PRE_CENSUS, shr, cr1, hw, hh, hw);
+            add_cost(0, cl1, cr1, nullptr, C1, hw, hh, D);
+            add_cost(1, cl2, cr2, C1, C2, W, H, D);
+            for (int k = 0; k < 8; k++) {
+                add_path(nullptr, nullptr, C1, hw, hh, D, kMsP2, kMsInvalid, (uint32_t)(Nh * D),
+                         kDirsMs[k][0], kDirsMs[k][1], kDirsMs[k][2], V1 + (size_t)k * Nh * D,
+                         ms_variant(D, false));
+                add_path(nullptr, nullptr, C2, W, H, D, kMsP2, kMsInvalid, (uint32_t)(Nh * D),
+                         kDirsMs[k][0], kDirsMs[k][1], kDirsMs[k][2], V2 + (size_t)k * Nh * D,
+                         ms_variant(D, true));
+            }
+            add_wta(V1, Nh * D, 8, (int)Nh, D, A + w.d1);
+            add_wta(V2, Nh * D, 8, (int)Nh, D, p.d_disp);
+        }
+    }
+    int rc = 0;
+    for (int L = 0; L < 2 && !rc; L++) {
+        BatchLane &ln = b->lane[L];
+        for (int k = 0; k < 3; k++)
+            ln.pre_n[k] = pre_items[L][k];
+        for (int k = 0; k < 2; k++)
+            ln.cost_n[k] = cost_items[L][k];
+        ln.wta_n = wta_items[L];
+        // path items in two orders: longest lines first, and pair by pair
+        std::vector<std::pair<long, uint32_t>> all;
+        for (size_t t = 0; t < ln.path.host.size(); t++) {
+            const BatchPathTask &tk = ln.path.host[t];
+            const int groups = (tk.dir.nlines + kLinesPerWave - 1) / kLinesPerWave;
+            for (int g = 0; g < groups; g++) {
+                const uint32_t code = ((uint32_t)t << 16) | (uint32_t)g;
+                ln.items_pair.push_back(code);
+                all.push_back({group_work(tk, g), code});
+            }
+        }
+        if (ln.path.host.size() >= 65536) {
+            set_err("sgmh_batch_plan: too many directions");
+            rc = -1;
+            break;
+        }
+        std::stable_sort(all.begin(), all.end(), [](const auto &a, const auto &c) { return a.first > c.first; });
+        for (auto &x : all)
+            ln.items_lpt.push_back(x.second);
+        for (int k = 0; k < 3 && !rc; k++)
+            rc = upload(ln.pre[k]);
+        for (int k = 0; k < 2 && !rc; k++)
+            rc = upload(ln.cost[k]);
+        if (!rc)
+            rc = upload(ln.path);
+        if (!rc)
+            rc = upload(ln.wta);
+        if (!rc && !ln.items_lpt.empty()) {
+            const size_t m = ln.items_lpt.size();
+            if (hipMalloc(&ln.d_items, 2 * m * sizeof(uint32_t)) != hipSuccess ||
+                hipMemcpy(ln.d_items, ln.items_lpt.data(), m * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(ln.d_items + m, ln.items_pair.data(), m * 4, hipMemcpyHostToDevice) != hipSuccess) {
+                set_err("sgmh_batch_plan: cannot upload the path work list");
+                rc = -1;
+            }
+        }
+        if (!rc && hipMalloc(&ln.d_counter, 64) != hipSuccess) {
+            set_err("sgmh_batch_plan: cannot allocate the work counter");
+            rc = -1;
+        }
+    }
+    if (!rc && (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
+                hipStreamCreateWithFlags(&b->cap2, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)) {
+        set_err("sgmh_batch_plan: cannot create streams / events");
+        rc = -1;
+    }
+    if (!rc && (flags & SGMH_BATCH_AUTOTUNE))
+        rc = autotune(b);
+    if (!rc && !(flags & SGMH_BATCH_NO_GRAPH))
+        rc = capture(b);
+    if (!rc && hipStreamSynchronize(b->cap) != hipSuccess) {
+        set_err("sgmh_batch_plan: stream synchronisation failed");
+        rc = -1;
+    }
+    if (rc) {
+        sgmh_batch_destroy(b);
+        return -1;
+    }
+    *out = b;
+    return 0;
+}
+
+int sgmh_batch_run(sgmh_batch *b, void *stream)
+{
+    REQUIRE(b, "sgmh_batch_run: NULL batch");
+    CK(hipSetDevice(b->ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    if (b->graph_ok) {
+        CK(hipGraphLaunch(b->exec, s));
+        return 0;
+    }
+    return enqueue_phases(b, s, b->sched);
+}
+
+int sgmh_batch_run_loop(sgmh_batch *b, void *stream)
+{
+    REQUIRE(b, "sgmh_batch_run_loop: NULL batch");
+    for (const sgmh_pair &p : b->pairs) {
+        const int rc = p.mode == SGMH_PLAIN
+                           ? sgmh_plain_dev(b->ctx, p.d_left, p.d_right, p.W, p.H, p.D, p.paths, p.d_disp, stream)
+                           : sgmh_ms_dev(b->ctx, p.d_left, p.d_right, p.W, p.H, p.D, p.d_disp, nullptr, stream);
+        if (rc)
+            return -1;
+    }
+    return 0;
+}
+
+int sgmh_batch_describe(const sgmh_batch *b, char *buf, size_t len)
+{
+    if (!b || !buf || len == 0)
+        return -1;
+    int plain = 0, ms = 0;
+    for (const sgmh_pair &p : b->pairs)
+        (p.mode == SGMH_PLAIN ? plain : ms)++;
+    const int n = snprintf(
+        buf, len,
+        "{\"pairs\":%zu,\"plain\":%d,\"multiscale\":%d,\"workspace_bytes\":%zu,\"path_items\":%zu,"
+        "\"path_tasks\":%zu,\"wta_tasks\":%zu,\"graph\":%s,\"schedule\":{\"order\":\"%s\","
+        "\"path_waves_per_simd\":%d},\"tuning\":%s}",
+        b->pairs.size(), plain, ms, b->arena_bytes, b->lane[0].items_lpt.size() + b->lane[1].items_lpt.size(),
+        b->lane[0].path.host.size() + b->lane[1].path.host.size(),
+        b->lane[0].wta.host.size() + b->lane[1].wta.host.size(), b->graph_ok ? "true" : "false",
+        b->sched.order == 0 ? "longest-first" : "pair-major", b->sched.path_waves_per_simd,
+        b->tuning.empty() ? "null" : b->tuning.c_str());
+    return n;
+}
+
+void sgmh_batch_destroy(sgmh_batch *b)
+{
+    if (!b)
+        return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    for (BatchLane &ln : b->lane) {
+        for (auto &a : ln.pre)
+            if (a.dev)
+                (void)hipFree(a.dev);
+        for (auto &a : ln.cost)
+            if (a.dev)
+                (void)hipFree(a.dev);
+        if (ln.path.dev)
+            (void)hipFree(ln.path.dev);
+        if (ln.wta.dev)
+            (void)hipFree(ln.wta.dev);
+        if (ln.d_items)
+            (void)hipFree(ln.d_items);
+        if (ln.d_counter)
+            (void)hipFree(ln.d_counter);
+    }
+    if (b->arena)
+        (void)hipFree(b->arena);
+    if (b->cap)
+        (void)hipStreamDestroy(b->cap);
+    if (b->cap2)
+        (void)hipStreamDestroy(b->cap2);
+    if (b->fork)
+        (void)hipEventDestroy(b->fork);
+    if (b->join)
+        (void)hipEventDestroy(b->join);
+    delete b;
+}
+
+} // extern "C"

@@ -1,0 +1,128 @@
+// sgm_internal.h -- host-side plumbing shared by the single-pair and batch
+// entry points of libsgm_hip: error reporting, device workspaces, the context.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+#include "sgm_hip.h"
+
+// thread-local last error, read back through sgmh_last_error()
+void sgmh_set_error(const char *fmt, ...);
+#define set_err sgmh_set_error
+
+#define CK(call)                                                                     \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            set_err("HIP %s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+            return -1;                                                               \
+        }                                                                            \
+    } while (0)
+
+#define REQUIRE(cond, ...)                                                           \
+    do {                                                                             \
+        if (!(cond)) {                                                               \
+            set_err(__VA_ARGS__);                                                    \
+            return -1;                                                               \
+        }                                                                            \
+    } while (0)
+
+enum Phase { PH_CENSUS = 0, PH_COST, PH_AGG, PH_ARGMIN, PH_XFER, PH_N };
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct sgmh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, std::pair<int, int>>> marks; // phase, ev begin, ev end
+    hipStream_t last_stream = nullptr;
+    // workspaces
+    DevBuf img_l, img_r, disp, disp1;          // host-buffer entry points
+    DevBuf cl, cr;                             // census codes
+    DevBuf vol;                                // direction volumes
+    DevBuf half_l, half_r, sob_l, sob_r;       // multiscale images
+    DevBuf cl1, cr1, cost1, cost2, vol1;       // multiscale stage buffers
+};
+
+inline int ensure(DevBuf &b, size_t bytes)
+{
+    if (b.cap >= bytes)
+        return 0;
+    if (b.p) {
+        CK(hipDeviceSynchronize());
+        CK(hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    CK(hipMalloc(&b.p, bytes));
+    b.cap = bytes;
+    return 0;
+}
+
+inline void free_buf(DevBuf &b)
+{
+    if (b.p)
+        (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// Phase timing: events are recorded only when timing is on.
+struct PhaseScope {
+    sgmh_ctx *ctx;
+    hipStream_t s;
+    int phase, ev0 = -1;
+    PhaseScope(sgmh_ctx *c, hipStream_t st, int ph) : ctx(c), s(st), phase(ph)
+    {
+        if (ctx->timing)
+            ev0 = record();
+    }
+    int record()
+    {
+        if (ctx->ev_used == ctx->events.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess)
+                return -1;
+            ctx->events.push_back(e);
+        }
+        const int i = (int)ctx->ev_used++;
+        if (hipEventRecord(ctx->events[i], s) != hipSuccess)
+            return -1;
+        return i;
+    }
+    ~PhaseScope()
+    {
+        if (ctx->timing && ev0 >= 0) {
+            const int ev1 = record();
+            if (ev1 >= 0)
+                ctx->marks.push_back({phase, {ev0, ev1}});
+        }
+    }
+};
+
+inline void begin_call(sgmh_ctx *ctx, hipStream_t s)
+{
+    ctx->ev_used = 0;
+    ctx->marks.clear();
+    ctx->last_stream = s;
+}
+
+inline int check_dims(int W, int H, int D)
+{
+    REQUIRE(W >= 16 && H >= 16 && W <= 16384 && H <= 16384,
+            "sgmh: image %dx%d outside 16..16384", W, H);
+    REQUIRE(D == 16 || D == 32 || D == 64 || D == 128 || D == 256,
+            "sgmh: D=%d unsupported (16, 32, 64, 128 or 256)", D);
+    return 0;
+}
+

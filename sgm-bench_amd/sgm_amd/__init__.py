@@ -76,6 +76,16 @@ class SceneSpec(ctypes.Structure):
                 ("noise", _I), ("slanted", _I)]
 
 
+class Pair(ctypes.Structure):
+    """sgmh_pair (include/sgm_hip.h)."""
+    _fields_ = [("d_left", _P), ("d_right", _P), ("W", _I), ("H", _I), ("D", _I),
+                ("mode", _I), ("paths", _I), ("d_disp", _P)]
+
+
+PLAIN, MULTISCALE = 0, 1
+BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
+
+
 def hip_lib() -> ctypes.CDLL:
     global _hip
     if _hip is None:
@@ -95,9 +105,17 @@ def hip_lib() -> ctypes.CDLL:
         h.sgmh_half_dev.argtypes = [_P, _P, _I, _I, _P, _P]
         h.sgmh_path_plain_dev.argtypes = [_P, _P, _P, _I, _I, _I, _I, _I, _P, _P]
         h.sgmh_copy_probe.argtypes = [_P, _P, _P, ctypes.c_size_t, _P]
+        h.sgmh_batch_plan.argtypes = [_P, ctypes.POINTER(Pair), _I, _I, ctypes.POINTER(_P)]
+        h.sgmh_batch_run.argtypes = [_P, _P]
+        h.sgmh_batch_run_loop.argtypes = [_P, _P]
+        h.sgmh_batch_describe.argtypes = [_P, ctypes.c_char_p, ctypes.c_size_t]
+        h.sgmh_batch_destroy.argtypes = [_P]
+        h.sgmh_batch_destroy.restype = None
         for name in ("sgmh_ctx_create", "sgmh_set_timing", "sgmh_get_times", "sgmh_plain",
                      "sgmh_plain_dev", "sgmh_ms", "sgmh_ms_dev", "sgmh_census_dev",
-                     "sgmh_sobel_dev", "sgmh_half_dev", "sgmh_path_plain_dev", "sgmh_copy_probe"):
+                     "sgmh_sobel_dev", "sgmh_half_dev", "sgmh_path_plain_dev", "sgmh_copy_probe",
+                     "sgmh_batch_plan", "sgmh_batch_run", "sgmh_batch_run_loop",
+                     "sgmh_batch_describe"):
             getattr(h, name).restype = _I
         _hip = h
     return _hip
@@ -148,6 +166,41 @@ WORKLOADS = {
     "B": dict(W=1920, H=800, D=64, planes=(20, 20), occl_permille=50, seed=31),
     "C": dict(W=1920, H=1080, D=64, planes=(20, 20), occl_permille=50, seed=41),
 }
+
+
+# The batched workload: 24 pairs of uneven size, D and configuration in one
+# submission -- 8 x 640x480 D=64 plain, 8 x 1280x720 D=64 plain,
+# 4 x 1920x800 D=128 multiscale 8-path, 4 x 1920x1080 D=128 plain.
+def batch_spec():
+    out = []
+    for i in range(8):
+        out.append(dict(name=f"p640_{i}", W=640, H=480, D=64, mode=PLAIN, paths=4,
+                        scene=dict(kind="planes", W=640, H=480, dmax=63, seed=200 + i,
+                                   planes=[8, 12], occl=0, noise=2, slanted=0)))
+    for i in range(8):
+        out.append(dict(name=f"p1280_{i}", W=1280, H=720, D=64, mode=PLAIN, paths=4,
+                        scene=dict(kind="planes", W=1280, H=720, dmax=63, seed=210 + i,
+                                   planes=[8, 12], occl=20, noise=2, slanted=0)))
+    for i in range(4):
+        out.append(dict(name=f"ms1920x800_{i}", W=1920, H=800, D=128, mode=MULTISCALE, paths=8,
+                        scene=dict(kind="planes", W=1920, H=800, dmax=127, seed=220 + i,
+                                   planes=[20, 20], occl=50, noise=2, slanted=0)))
+    for i in range(4):
+        out.append(dict(name=f"p1080_{i}", W=1920, H=1080, D=128, mode=PLAIN, paths=4,
+                        scene=dict(kind="planes", W=1920, H=1080, dmax=127, seed=230 + i,
+                                   planes=[20, 20], occl=50, noise=2, slanted=0)))
+    return out
+
+
+def make_scene(spec: dict):
+    """A scene from its recorded generator call (the fixtures' "scene" dicts)."""
+    if spec["kind"] == "reference":
+        L, R, _ = scene_reference(spec["W"], spec["H"], spec["gen_D"], spec["seed"])
+    else:
+        L, R, _ = scene_planes(spec["W"], spec["H"], spec["dmax"], spec["seed"],
+                               planes=tuple(spec["planes"]), occl_permille=spec["occl"],
+                               noise=spec["noise"], slanted=bool(spec["slanted"]))
+    return L, R
 
 
 def workload_scene(name: str):
@@ -268,6 +321,9 @@ class Context:
     def half_dev(self, d_img: int, W: int, H: int, d_out: int, stream: int = 0) -> None:
         self._ck(self._lib.sgmh_half_dev(self._h, d_img, W, H, d_out, stream or None))
 
+    def batch(self, pairs, flags: int = BATCH_AUTOTUNE) -> "Batch":
+        return Batch(self, pairs, flags)
+
     def copy_probe(self, d_dst: int, d_src: int, nbytes: int, stream: int = 0) -> None:
         self._ck(self._lib.sgmh_copy_probe(self._h, d_dst, d_src, nbytes, stream or None))
 
@@ -275,3 +331,42 @@ class Context:
                        d_L: int, stream: int = 0) -> None:
         self._ck(self._lib.sgmh_path_plain_dev(self._h, d_cl, d_cr, W, H, D, dx, dy, d_L,
                                                stream or None))
+
+
+class Batch:
+    """A planned batch (sgmh_batch): pairs = list of dicts with keys d_left,
+    d_right, d_disp (device addresses), W, H, D, mode, paths."""
+
+    def __init__(self, ctx: Context, pairs, flags: int = BATCH_AUTOTUNE):
+        self._ctx = ctx
+        self._lib = ctx._lib
+        arr = (Pair * len(pairs))()
+        for i, p in enumerate(pairs):
+            arr[i] = Pair(p["d_left"], p["d_right"], p["W"], p["H"], p["D"], p["mode"],
+                          p["paths"], p["d_disp"])
+        h = _P()
+        ctx._ck(self._lib.sgmh_batch_plan(ctx.handle, arr, len(pairs), flags, ctypes.byref(h)))
+        self._h = h
+
+    def run(self, stream: int = 0) -> None:
+        self._ctx._ck(self._lib.sgmh_batch_run(self._h, stream or None))
+
+    def run_loop(self, stream: int = 0) -> None:
+        self._ctx._ck(self._lib.sgmh_batch_run_loop(self._h, stream or None))
+
+    def describe(self) -> dict:
+        import json
+        buf = ctypes.create_string_buffer(65536)
+        self._lib.sgmh_batch_describe(self._h, buf, len(buf))
+        return json.loads(buf.value.decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sgmh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

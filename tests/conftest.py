@@ -24,13 +24,7 @@ def goldens():
 
 def make_scene(spec):
     import sgm_amd
-    if spec["kind"] == "reference":
-        L, R, _ = sgm_amd.scene_reference(spec["W"], spec["H"], spec["gen_D"], spec["seed"])
-    else:
-        L, R, _ = sgm_amd.scene_planes(spec["W"], spec["H"], spec["dmax"], spec["seed"],
-                                       planes=tuple(spec["planes"]), occl_permille=spec["occl"],
-                                       noise=spec["noise"], slanted=bool(spec["slanted"]))
-    return L, R
+    return sgm_amd.make_scene(spec)
 
 
 @pytest.fixture(scope="session")

@@ -142,6 +142,21 @@ def main():
                                     "input_fnv": fnv(np.concatenate([L.ravel(), R.ravel()])),
                                     "fnv": h})
             print(name, h, flush=True)
+        # the batched workload: compiled reference for the plain pairs; the
+        # oracle restatement for the D=128 multiscale pairs (the reference's
+        # multiscale oracle is D=64 only, multiscale/sgm_ms_ref.c:38)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        out["batch"] = []
+        for p in sgm_amd.batch_spec():
+            L, R = scene(p["scene"])
+            if p["mode"] == sgm_amd.PLAIN:
+                h, src = run_plain(L, R, p["D"], p["paths"], tmp), "reference"
+            else:
+                h, src = oracle.fnv1a64(oracle.sgm_ms(L, R, p["D"])), "oracle (no D=128 reference)"
+            out["batch"].append({"name": p["name"], "scene": p["scene"], "mode": "plain" if p["mode"] == 0 else "ms",
+                                 "D": p["D"], "paths": p["paths"], "fnv": h, "source": src})
+            print(p["name"], h, src, flush=True)
     with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "goldens.json"), "w") as f:
         json.dump(out, f, indent=1)
 

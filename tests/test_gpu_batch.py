@@ -1,0 +1,94 @@
+"""Batched mode: every pair of a mixed submission is bit-exact to its oracle,
+through the captured graph, on replays, and through the single-pair loop."""
+import numpy as np
+import pytest
+
+import oracle
+import sgm_amd
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def upload(pairs_np):
+    """pairs_np: list of (spec, L, R) -> (pair dicts for the ABI, keep-alive tensors)."""
+    keep, pairs = [], []
+    for spec, L, R in pairs_np:
+        dl, dr = torch.from_numpy(L).cuda(), torch.from_numpy(R).cuda()
+        oh, ow = (spec["H"], spec["W"]) if spec["mode"] == sgm_amd.PLAIN else (spec["H"] // 2, spec["W"] // 2)
+        do = torch.full((oh, ow), 0xEE, dtype=torch.uint8, device="cuda")
+        keep += [dl, dr, do]
+        pairs.append(dict(d_left=dl.data_ptr(), d_right=dr.data_ptr(), d_disp=do.data_ptr(),
+                          W=spec["W"], H=spec["H"], D=spec["D"], mode=spec["mode"], paths=spec["paths"]))
+    return pairs, keep
+
+
+def want(spec, L, R):
+    if spec["mode"] == sgm_amd.PLAIN:
+        return oracle.sgm_plain(L, R, spec["D"], spec["paths"])
+    return oracle.sgm_ms(L, R, spec["D"])
+
+
+SMALL = [
+    dict(W=160, H=120, D=64, mode=sgm_amd.PLAIN, paths=4, seed=1),
+    dict(W=97, H=61, D=64, mode=sgm_amd.PLAIN, paths=8, seed=2),
+    dict(W=200, H=90, D=128, mode=sgm_amd.PLAIN, paths=4, seed=3),
+    dict(W=64, H=40, D=32, mode=sgm_amd.PLAIN, paths=8, seed=4),
+    dict(W=128, H=96, D=64, mode=sgm_amd.MULTISCALE, paths=8, seed=5),
+    dict(W=160, H=64, D=128, mode=sgm_amd.MULTISCALE, paths=8, seed=6),
+    dict(W=62, H=34, D=32, mode=sgm_amd.MULTISCALE, paths=8, seed=7),
+]
+
+
+@pytest.mark.parametrize("flags", [sgm_amd.BATCH_AUTOTUNE, sgm_amd.BATCH_NO_GRAPH, 0])
+def test_mixed_batch_matches_oracle(ctx, flags):
+    data = []
+    for sp in SMALL:
+        L, R, _ = sgm_amd.scene_planes(sp["W"], sp["H"], min(sp["D"], sp["W"]) - 1, sp["seed"], planes=(3, 6))
+        data.append((sp, L, R))
+    pairs, keep = upload(data)
+    b = ctx.batch(pairs, flags)
+    s = torch.cuda.current_stream().cuda_stream
+    wants = [want(sp, L, R) for sp, L, R in data]
+    for rep in range(3):  # graph replays, with eager work in between
+        for t in keep[2::3]:
+            t.fill_(0xEE)
+        b.run(s)
+        torch.cuda.synchronize()
+        _ = torch.ones(1000, device="cuda").sum().item()
+        for (sp, _, _), w, d_out in zip(data, wants, keep[2::3]):
+            got = d_out.cpu().numpy()
+            assert np.array_equal(got, w), f"replay {rep}: {sp}: {(got != w).sum()} pixels differ"
+    for t in keep[2::3]:
+        t.fill_(0xEE)
+    b.run_loop(s)
+    torch.cuda.synchronize()
+    for (sp, _, _), w, d_out in zip(data, wants, keep[2::3]):
+        assert np.array_equal(d_out.cpu().numpy(), w), f"loop: {sp}"
+    info = b.describe()
+    assert info["pairs"] == len(SMALL)
+    b.close()
+
+
+def test_full_batch_goldens(ctx, goldens):
+    data, specs = [], sgm_amd.batch_spec()
+    for sp in specs:
+        L, R = sgm_amd.make_scene(sp["scene"])
+        data.append((sp, L, R))
+    pairs, keep = upload(data)
+    b = ctx.batch(pairs, sgm_amd.BATCH_AUTOTUNE)
+    b.run(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    gold = {g["name"]: g["fnv"] for g in goldens["batch"]}
+    for sp, d_out in zip(specs, keep[2::3]):
+        assert sgm_amd.fnv1a64(d_out.cpu().numpy()) == gold[sp["name"]], sp["name"]
+    print(b.describe())
+    b.close()
+
+
+def test_batch_rejects_unsupported(ctx):
+    L = torch.zeros((64, 64), dtype=torch.uint8, device="cuda")
+    bad = dict(d_left=L.data_ptr(), d_right=L.data_ptr(), d_disp=L.data_ptr(), W=64, H=64, D=256,
+               mode=sgm_amd.PLAIN, paths=4)
+    with pytest.raises(sgm_amd.SgmError, match="batched mode"):
+        ctx.batch([bad])
