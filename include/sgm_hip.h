@@ -109,6 +109,46 @@ int sgmh_half_dev(sgmh_ctx *ctx, const uint8_t *d_img, int W, int H,
 int sgmh_path_plain_dev(sgmh_ctx *ctx, const uint64_t *d_cl, const uint64_t *d_cr,
                         int W, int H, int D, int dx, int dy, uint8_t *d_L, void *stream);
 
+/* ---- windowed SGM: the learned path's reduced search (no reference
+ *      counterpart).  Plain-path rules (4 paths, P1=20, P2=192, invalid 62)
+ *      restricted per 16x16 tile to the 16-disparity blocks
+ *      [blk[t] & 15, blk[t] >> 4]; tiles are row-major, ceil(W/16) per row.
+ *      With every block active the output equals sgmh_plain's.  D = 32, 64 or
+ *      128. ---- */
+int sgmh_window(sgmh_ctx *ctx, const uint8_t *left, const uint8_t *right, int W, int H, int D,
+                const uint8_t *blk, uint8_t *disp, sgmh_times *t);
+int sgmh_window_dev(sgmh_ctx *ctx, const uint8_t *d_left, const uint8_t *d_right, int W, int H,
+                    int D, const uint8_t *d_blk, uint8_t *d_disp, void *stream);
+
+/* ---- the learned path's window-proposal network (no reference
+ *      counterpart): 4x-downsampled matching features, 6 conv layers
+ *      3x3 32->32 (bf16 MFMA, fp32 accumulation), per 16x16 tile a softmax over
+ *      32 disparity bins of 4 (D = 128), the 32-disparity window (two 16-blocks)
+ *      with the most mass, that mass as the confidence.  Frames need W and H
+ *      multiples of 4.  A network owns its parameters (fp32 master copy, bf16
+ *      packed copies), Adam state and activation buffers. ---- */
+typedef struct sgmh_cnn sgmh_cnn;
+size_t sgmh_cnn_num_params(void);
+int sgmh_cnn_create(sgmh_ctx *ctx, sgmh_cnn **out);
+void sgmh_cnn_destroy(sgmh_cnn *net);
+/* fp32 parameters, per layer W[co][ci][tap] (tap = 3*ky + kx) then b[co];
+ * setting them also resets the optimiser state */
+int sgmh_cnn_set_params(sgmh_cnn *net, const float *theta, size_t n);
+int sgmh_cnn_get_params(sgmh_cnn *net, float *theta, size_t n);
+/* Per-tile block ranges for sgmh_window_dev (b0 | b1 << 4, ceil(W/16) per row):
+ * the best window where its confidence >= thresh, else all 8 blocks. */
+int sgmh_cnn_propose_dev(sgmh_cnn *net, const uint8_t *d_left, const uint8_t *d_right, int W, int H,
+                         float thresh, uint8_t *d_blk, float *d_conf, void *stream);
+/* One Adam step (lr > 0) on one frame against target disparities (the plain
+ * full-D SGM output); lr <= 0 computes the gradients only.  *loss (may be NULL)
+ * receives the tile cross-entropy before the step. */
+int sgmh_cnn_train_step_dev(sgmh_cnn *net, const uint8_t *d_left, const uint8_t *d_right,
+                            const uint8_t *d_target, int W, int H, float lr, float *loss, void *stream);
+int sgmh_cnn_get_grads(sgmh_cnn *net, float *grad, size_t n);
+/* features (bf16 bits, [H/4][W/4][32]) and logits (fp32, same shape) of the
+ * last forward pass; either may be NULL */
+int sgmh_cnn_get_activations(sgmh_cnn *net, uint16_t *features, float *logits);
+
 /* ---- batched mode: many pairs of uneven size, D and configuration in one
  *      submission (no reference counterpart; the reference runs one pair per
  *      binary invocation, common/harness.c:150-158) ---- */

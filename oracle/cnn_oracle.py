@@ -1,0 +1,204 @@
+"""cnn_oracle.py -- float reference of the window-proposal network.
+
+TEST INFRASTRUCTURE ONLY (the checker for the HIP CNN kernels; see
+oracle/oracle.py for what may use the oracle).  The network has no reference
+counterpart: it is defined here and by its HIP implementation together.
+
+Definition (D = 128, frames with W and H multiples of 4):
+  features   Ld, Rd = 4x4 block sums of the left / right images (h = H/4,
+             w = W/4).  For bin k in 0..31 (full-resolution disparities
+             4k .. 4k+3):  f_k(x, y) = sum over the clamped 3x3 neighbourhood
+             (x', y') of |Ld(x', y') - Rd(x' - k, y')|, with the difference
+             taken as 4080 (= 16*255) where x' - k < 0, divided by 9*4080;
+             rounded to bf16.  Layout [h][w][32].
+  layers     6 x (3x3 conv, 32 -> 32 channels, zero padding, fp32 bias);
+             layers 1-5 ReLU and bf16 output, layer 6 linear fp32 logits.
+             Inputs and weights enter the products as bf16, sums in fp32.
+  head       16x16-pixel tile t = 4x4 low-resolution pixels (fewer at the
+             bottom / right edge): z_t = mean of their logits, p_t =
+             softmax(z_t).  Window start b0 (16-disparity blocks) maximises
+             conf = sum of p over bins 4*b0 .. 4*b0+7; below the confidence
+             threshold the tile falls back to all blocks.
+  loss       cross-entropy -sum_k q_tk log p_tk, averaged over tiles, with q_t
+             the histogram of the tile's target disparities in bins of 4.
+Parameters are stored flat in fp32: for each layer W[co][ci][tap] (tap =
+3*ky + kx) then b[co].
+"""
+from __future__ import annotations
+
+import numpy as np
+
+C = 32          # channels / bins
+ROUND = True    # bf16 roundings on; off only for finite-difference checks
+LAYERS = 6
+NPARAM = LAYERS * (C * C * 9 + C)
+
+
+def ft():
+    """Working float type: float32 (the device's), float64 with roundings off."""
+    return np.float32 if ROUND else np.float64
+
+
+def bf16(a):
+    """Round float32 to bfloat16 (nearest, ties to even), returned as float32."""
+    if not ROUND:
+        return np.ascontiguousarray(a, np.float64)
+    a = np.ascontiguousarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def split_params(theta):
+    out, o = [], 0
+    for _ in range(LAYERS):
+        w = theta[o:o + C * C * 9].reshape(C, C, 9)
+        o += C * C * 9
+        b = theta[o:o + C]
+        o += C
+        out.append((w, b))
+    return out
+
+
+def init_params(seed: int) -> np.ndarray:
+    """He-normal weights, zero biases (the same stream is not required on the
+    GPU; weights are set explicitly there)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for _ in range(LAYERS):
+        parts.append(rng.normal(0, np.sqrt(2.0 / (C * 9)), C * C * 9).astype(np.float32))
+        parts.append(np.zeros(C, np.float32))
+    return np.concatenate(parts)
+
+
+def features(left, right):
+    L = np.asarray(left, np.int64)
+    R = np.asarray(right, np.int64)
+    H, W = L.shape
+    h, w = H // 4, W // 4
+    Ld = L.reshape(h, 4, w, 4).sum(axis=(1, 3))
+    Rd = R.reshape(h, 4, w, 4).sum(axis=(1, 3))
+    f = np.empty((h, w, C), ft())
+    xs = np.arange(w)
+    for k in range(C):
+        diff = np.full((h, w), 4080, np.int64)
+        ok = xs >= k
+        diff[:, ok] = np.abs(Ld[:, ok] - Rd[:, xs[ok] - k])
+        pad = np.pad(diff, 1, mode="edge")
+        acc = np.zeros((h, w), np.int64)
+        for dy in range(3):
+            for dx in range(3):
+                acc += pad[dy:dy + h, dx:dx + w]
+        f[:, :, k] = acc.astype(ft()) / ft()(9 * 4080)
+    return bf16(f)
+
+
+def conv(x, w, b):
+    """3x3 'same' conv, x [h][w][ci] (bf16 values), w [co][ci][9]; fp32 out."""
+    h, wd, _ = x.shape
+    xp = np.pad(x, ((1, 1), (1, 1), (0, 0)))
+    wb = bf16(w)
+    y = np.zeros((h, wd, w.shape[0]), ft())
+    for t in range(9):
+        ky, kx = divmod(t, 3)
+        y += xp[ky:ky + h, kx:kx + wd, :] @ wb[:, :, t].T
+    return y + b
+
+
+def forward(theta, x0):
+    acts = [x0]
+    x = x0
+    pre = []
+    for i, (w, b) in enumerate(split_params(theta)):
+        z = conv(x, w, b)
+        pre.append(z)
+        if i < LAYERS - 1:
+            x = bf16(np.maximum(z, 0))
+            acts.append(x)
+        else:
+            x = z
+    return acts, x   # acts[i] = input of layer i; x = logits
+
+
+def tile_logits(logits):
+    h, w, _ = logits.shape
+    th, tw = (h + 3) // 4, (w + 3) // 4
+    z = np.zeros((th, tw, C), ft())
+    n = np.zeros((th, tw, 1), ft())
+    for y in range(h):
+        for x in range(w):
+            z[y // 4, x // 4] += logits[y, x]
+            n[y // 4, x // 4] += 1
+    return z / n, n
+
+
+def softmax(z):
+    e = np.exp(z - z.max(axis=-1, keepdims=True))
+    return e / e.sum(axis=-1, keepdims=True)
+
+
+def targets(disp, H, W):
+    """Per-tile histogram of target disparities in bins of 4 (D = 128)."""
+    th, tw = (H + 15) // 16, (W + 15) // 16
+    q = np.zeros((th, tw, C), ft())
+    d = np.asarray(disp, np.int64) // 4
+    for y in range(H):
+        np.add.at(q[y // 16], (np.arange(W) // 16, d[y]), 1)
+    return q / q.sum(axis=-1, keepdims=True)
+
+
+def loss_and_grads(theta, left, right, disp):
+    """Mean tile cross-entropy and its gradient w.r.t. theta (fp32 backprop;
+    the bf16 roundings of the forward are treated as identity)."""
+    theta = np.asarray(theta, ft())
+    H, W = np.asarray(left).shape
+    x0 = features(left, right)
+    acts, logits = forward(theta, x0)
+    z, n = tile_logits(logits)
+    p = softmax(z)
+    q = targets(disp, H, W)
+    T = z.shape[0] * z.shape[1]
+    loss = float(-(q * np.log(np.maximum(p, 1e-30))).sum() / T)
+    dz = (p - q) / T
+    h, w, _ = logits.shape
+    dy = np.empty_like(logits)
+    for y in range(h):
+        for x in range(w):
+            dy[y, x] = dz[y // 4, x // 4] / n[y // 4, x // 4]
+    params = split_params(theta)
+    grads = [None] * LAYERS
+    for i in range(LAYERS - 1, -1, -1):
+        w, b = params[i]
+        x = acts[i]
+        hh, ww, _ = x.shape
+        xp = np.pad(x, ((1, 1), (1, 1), (0, 0)))
+        gw = np.zeros_like(w)
+        for t in range(9):
+            ky, kx = divmod(t, 3)
+            gw[:, :, t] = dy.reshape(-1, C).T @ xp[ky:ky + hh, kx:kx + ww, :].reshape(-1, C)
+        gb = dy.sum(axis=(0, 1))
+        grads[i] = (gw, gb)
+        if i > 0:
+            dx = np.zeros((hh + 2, ww + 2, C), ft())
+            wb = bf16(w)
+            for t in range(9):
+                ky, kx = divmod(t, 3)
+                dx[ky:ky + hh, kx:kx + ww, :] += dy @ wb[:, :, t]
+            dx = dx[1:-1, 1:-1, :]
+            dy = dx * (acts[i] > 0)
+    flat = []
+    for gw, gb in grads:
+        flat += [gw.ravel(), gb]
+    return loss, np.concatenate(flat).astype(ft())
+
+
+def propose(theta, left, right, thresh=0.9):
+    """Per-tile block ranges (b0 | b1 << 4) and confidences, D = 128."""
+    _, logits = forward(theta, features(left, right))
+    z, _ = tile_logits(logits)
+    p = softmax(z)
+    win = np.stack([p[..., 4 * b:4 * b + 8].sum(-1) for b in range(7)], -1)
+    b0 = win.argmax(-1)
+    conf = win.max(-1)
+    blk = np.where(conf >= thresh, b0 | ((b0 + 1) << 4), 7 << 4).astype(np.uint8)
+    return blk, conf.astype(np.float32)

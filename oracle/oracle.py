@@ -43,8 +43,9 @@ def lib() -> ctypes.CDLL:
         L.orc_half.argtypes = [_P, _I, _I, _P]
         L.orc_sgm_ms.argtypes = [_P, _P, _I, _I, _I, _P, _P]
         L.orc_sgm_ms_omp.argtypes = [_P, _P, _I, _I, _I, _P, _P, _I]
+        L.orc_sgm_window.argtypes = [_P, _P, _I, _I, _I, _I, _P, _P]
         for n in ("orc_sgm_plain", "orc_sgm_plain_omp", "orc_sgm_ms", "orc_sgm_ms_omp",
-                  "orc_omp_max_threads"):
+                  "orc_omp_max_threads", "orc_sgm_window"):
             getattr(L, n).restype = _I
         _lib = L
     return _lib
@@ -148,3 +149,15 @@ def half(img) -> np.ndarray:
 
 def omp_threads() -> int:
     return lib().orc_omp_max_threads()
+
+
+def sgm_window(left, right, blk, D: int = 128, paths: int = 4) -> np.ndarray:
+    """Windowed SGM; blk[ty, tx] = b0 | b1 << 4 (active 16-blocks, inclusive)."""
+    L, R = _c(left), _c(right)
+    H, W = L.shape
+    blk = _c(blk)
+    assert blk.shape == ((H + 15) // 16, (W + 15) // 16)
+    out = np.empty((H, W), np.uint8)
+    if lib().orc_sgm_window(_p(L), _p(R), W, H, D, paths, _p(blk), _p(out)):
+        raise ValueError("oracle rejected the arguments")
+    return out

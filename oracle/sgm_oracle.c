@@ -196,6 +196,8 @@ static void argmin_omp(const uint16_t *S, size_t npix, int D, uint8_t *disp)
 
 /* ---- Configuration A: common/sgm_ref.c:101-144 ---- */
 #define PLAIN_P1 20      /* sgm_params.h:23 */
+#define P_WIN1 PLAIN_P1
+#define P_WIN2 PLAIN_P2
 #define PLAIN_P2 192     /* sgm_params.h:24 */
 #define PLAIN_INVALID 62 /* sgm_params.h:27-29: 9*7-1 */
 
@@ -407,4 +409,90 @@ int orc_sgm_ms_omp(const uint8_t *left, const uint8_t *right, int W, int H, int 
     (void)nthreads;
 #endif
     return sgm_ms_impl(left, right, W, H, D, disp, disp1, 1);
+}
+
+/* ---- windowed SGM (build-defined, see sgm_oracle.h) ---- */
+#define WIN_ABSENT 100000
+
+static inline int tile_lo(const uint8_t *blk, int tiles_x, int x, int y)
+{
+    return 16 * (blk[(y >> 4) * tiles_x + (x >> 4)] & 15);
+}
+static inline int tile_hi(const uint8_t *blk, int tiles_x, int x, int y)
+{
+    return 16 * ((blk[(y >> 4) * tiles_x + (x >> 4)] >> 4) + 1);
+}
+
+int orc_sgm_window(const uint8_t *left, const uint8_t *right, int W, int H, int D,
+                   int paths, const uint8_t *blk, uint8_t *disp)
+{
+    if (W <= 0 || H <= 0 || D <= 0 || D > 256 || D % 16 || (paths != 4 && paths != 8))
+        return -1;
+    const int tiles_x = (W + 15) / 16, tiles_y = (H + 15) / 16, nb = D / 16;
+    for (int t = 0; t < tiles_x * tiles_y; t++)
+        if ((blk[t] & 15) > (blk[t] >> 4) || (blk[t] >> 4) >= nb)
+            return -1;
+    const size_t N = (size_t)W * H;
+    uint64_t *cl = (uint64_t *)malloc(N * 8), *cr = (uint64_t *)malloc(N * 8);
+    uint8_t *C = (uint8_t *)malloc(N * D), *L = (uint8_t *)malloc(N * D);
+    uint16_t *S = (uint16_t *)calloc(N * D, 2);
+    int rc = -1;
+    if (!cl || !cr || !C || !L || !S)
+        goto out;
+    orc_census9x7(left, W, H, cl);
+    orc_census9x7(right, W, H, cr);
+    orc_cost(cl, cr, W, H, D, PLAIN_INVALID, C);
+    for (int p = 0; p < paths; p++) {
+        const int dx = orc_dirs_plain[p][0], dy = orc_dirs_plain[p][1];
+        for (int yi = 0; yi < H; yi++) {
+            const int y = dy < 0 ? H - 1 - yi : yi;
+            for (int xi = 0; xi < W; xi++) {
+                const int x = dx < 0 ? W - 1 - xi : xi;
+                const size_t o = ((size_t)y * W + x) * D;
+                const int lo = tile_lo(blk, tiles_x, x, y), hi = tile_hi(blk, tiles_x, x, y);
+                const int px = x - dx, py = y - dy;
+                if (!(px >= 0 && px < W && py >= 0 && py < H)) {
+                    for (int d = lo; d < hi; d++)
+                        L[o + d] = C[o + d];
+                    continue;
+                }
+                const size_t po = ((size_t)py * W + px) * D;
+                const int plo = tile_lo(blk, tiles_x, px, py), phi = tile_hi(blk, tiles_x, px, py);
+                int mn = WIN_ABSENT;
+                for (int d = plo; d < phi; d++)
+                    if (L[po + d] < mn)
+                        mn = L[po + d];
+                for (int d = lo; d < hi; d++) {
+                    const int self = d >= plo && d < phi ? L[po + d] : WIN_ABSENT;
+                    const int below = d - 1 >= plo && d - 1 < phi ? L[po + d - 1] + P_WIN1 : WIN_ABSENT;
+                    const int above = d + 1 >= plo && d + 1 < phi ? L[po + d + 1] + P_WIN1 : WIN_ABSENT;
+                    int m = self;
+                    if (below < m) m = below;
+                    if (above < m) m = above;
+                    if (mn + P_WIN2 < m) m = mn + P_WIN2;
+                    if (m > 255) m = 255; /* every candidate saturates at 255 */
+                    L[o + d] = (uint8_t)(C[o + d] + m - mn);
+                }
+            }
+        }
+        for (size_t i = 0; i < N; i++) {
+            const int x = (int)(i % W), y = (int)(i / W);
+            const int lo = tile_lo(blk, tiles_x, x, y), hi = tile_hi(blk, tiles_x, x, y);
+            for (int d = lo; d < hi; d++)
+                S[i * D + d] = (uint16_t)(S[i * D + d] + L[i * D + d]);
+        }
+    }
+    for (size_t i = 0; i < N; i++) {
+        const int x = (int)(i % W), y = (int)(i / W);
+        const int lo = tile_lo(blk, tiles_x, x, y), hi = tile_hi(blk, tiles_x, x, y);
+        int best = lo;
+        for (int d = lo + 1; d < hi; d++)
+            if (S[i * D + d] < S[i * D + best])
+                best = d;
+        disp[i] = (uint8_t)best;
+    }
+    rc = 0;
+out:
+    free(cl); free(cr); free(C); free(L); free(S);
+    return rc;
 }

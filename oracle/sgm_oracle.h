@@ -24,6 +24,9 @@
  *                      hard-codes D=64 for the multiscale oracle
  *                      (multiscale/sgm_ms_ref.c:38). Pinned only by reduction to
  *                      the D=64 goldens and self-consistency.
+ *   orc_sgm_window     build-defined (no reference counterpart): pinned by the
+ *                      identity "all blocks active == orc_sgm_plain" and by a
+ *                      brute-force transition-matrix restatement in tests.
  */
 #ifndef SGM_ORACLE_H
 #define SGM_ORACLE_H
@@ -86,6 +89,19 @@ int orc_sgm_ms(const uint8_t *left, const uint8_t *right, int W, int H, int D,
                uint8_t *disp, uint8_t *disp1);
 int orc_sgm_ms_omp(const uint8_t *left, const uint8_t *right, int W, int H, int D,
                    uint8_t *disp, uint8_t *disp1, int nthreads);
+
+/* Windowed SGM (build-defined; no reference counterpart).  Tiles are 16x16
+ * pixels, tiles_x = ceil(W/16); tile t's active disparities are the 16-blocks
+ * [blk[t] & 15, blk[t] >> 4] (inclusive), i.e. d in [16*b0, 16*(b1+1)).
+ * Plain-path rules (P1=20, P2=192, invalid 62) with these changes: the
+ * predecessor's L at a disparity outside ITS active range is absent (+inf),
+ * mn is the min over its active range, and each candidate of m(d) saturates
+ * at 255 as in the reference step (sgm_ref.c:61-72).  S sums the 4 (or 8)
+ * directions over the pixel's active range; disp = argmin there, lowest d on
+ * ties.  With every tile's range = all blocks this is orc_sgm_plain exactly.
+ * paths = 4 or 8.  Returns 0 on success. */
+int orc_sgm_window(const uint8_t *left, const uint8_t *right, int W, int H, int D,
+                   int paths, const uint8_t *blk, uint8_t *disp);
 
 int orc_omp_max_threads(void);
 

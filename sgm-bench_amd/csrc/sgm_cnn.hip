@@ -1,0 +1,606 @@
+// sgm_cnn.hip -- the window-proposal network of the learned path: inference
+// and training on gfx950, bf16 MFMA with fp32 accumulation.
+//
+// Network (no reference counterpart; oracle/cnn_oracle.py is its float
+// restatement): features of a 4x-downsampled pair (32 bins of 3x3 absolute
+// differences of 4x4 block sums), 6 conv layers 3x3 32->32 (ReLU on 1-5,
+// fp32 logits from 6), per 16x16-pixel tile the mean of its 4x4 low-resolution
+// logits, softmax over 32 disparity bins (4 disparities each, D = 128), the
+// 32-disparity window (two 16-blocks) with the most probability mass and that
+// mass as the confidence.  Training minimises the tile cross-entropy against
+// the histogram of the plain full-D SGM disparities of the tile.
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_16x16x16bf16_1k: a wave owns
+// 16 consecutive pixels of a row (M) x 32 output channels (two 16-wide N
+// tiles); K = 9 taps x 32 input channels = 18 steps of 16.  The layer's packed
+// bf16 weights (the B operands of all 18 steps) stay in registers for the
+// wave's whole pixel loop; activations are NHWC bf16, 64 bytes per pixel, so
+// one A operand is one 8-byte load per lane.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#include <vector>
+
+#include "sgm_internal.h"
+
+namespace {
+
+constexpr int kC = 32;             // channels = disparity bins
+constexpr int kLayers = 6;
+constexpr int kWPer = kC * kC * 9; // weights per layer
+constexpr int kPPer = kWPer + kC;  // parameters per layer (weights + bias)
+constexpr int kNParam = kLayers * kPPer;
+constexpr int kDisp = 128;
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u = u + 0x7FFFu + ((u >> 16) & 1u); // round to nearest, ties to even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---- features ----
+__global__ __launch_bounds__(256) void k_blocksum(const uint8_t *__restrict__ img, int W, int H,
+                                                  int32_t *__restrict__ out)
+{
+    const int w = W / 4, h = H / 4;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= w * h)
+        return;
+    const int y = i / w, x = i - y * w;
+    int s = 0;
+    for (int dy = 0; dy < 4; dy++)
+        for (int dx = 0; dx < 4; dx++)
+            s += img[(size_t)(4 * y + dy) * W + 4 * x + dx];
+    out[i] = s;
+}
+
+// f_k = sum_{clamped 3x3} |Ld - Rd(x'-k)| (4080 where x'-k < 0) / (9*4080), bf16
+__global__ __launch_bounds__(256) void k_features(const int32_t *__restrict__ Ld, const int32_t *__restrict__ Rd,
+                                                  int w, int h, uint16_t *__restrict__ X)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= w * h * kC)
+        return;
+    const int k = i % kC, p = i / kC, y = p / w, x = p - y * w;
+    int acc = 0;
+    for (int dy = -1; dy <= 1; dy++) {
+        const int yy = min(max(y + dy, 0), h - 1);
+        for (int dx = -1; dx <= 1; dx++) {
+            const int xx = min(max(x + dx, 0), w - 1);
+            const int a = Ld[yy * w + xx];
+            acc += xx - k >= 0 ? abs(a - Rd[yy * w + xx - k]) : 4080;
+        }
+    }
+    X[i] = to_bf16((float)acc / (float)(9 * 4080));
+}
+
+// ---- convolution (implicit GEMM on MFMA) ----
+// mode 0: Y = relu(conv + b) as bf16      (hidden layers)
+// mode 1: Y = conv + b as fp32            (logits)
+// mode 2: Y = conv * (mask > 0) as bf16    (backward data through a ReLU)
+// Wp: packed bf16 B operands, [co][k] with k = tap*32 + ci, 32 x 288.
+__global__ __launch_bounds__(64) void k_conv(const uint16_t *__restrict__ X, int w, int h,
+                                             const uint16_t *__restrict__ Wp, const float *__restrict__ bias,
+                                             int mode, const uint16_t *__restrict__ mask, void *__restrict__ Y)
+{
+    const int l = threadIdx.x, lr = l & 15, lq = l >> 4;
+    s16x4 B[18][2];
+#pragma unroll
+    for (int kb = 0; kb < 18; kb++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(Wp + (size_t)(16 * nt + lr) * 288 + 16 * kb + 4 * lq);
+            B[kb][nt] = __builtin_bit_cast(s16x4, v);
+        }
+    const int tiles_x = (w + 15) / 16, ntiles = tiles_x * h;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int y = t / tiles_x, x0 = (t - y * tiles_x) * 16;
+        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 18; kb++) {
+            const int tap = kb >> 1, ky = tap / 3, kx = tap % 3, ci = (kb & 1) * 16 + 4 * lq;
+            const int xx = x0 + lr + kx - 1, yy = y + ky - 1;
+            uint2 v = make_uint2(0u, 0u);
+            if (xx >= 0 && xx < w && yy >= 0 && yy < h)
+                v = *reinterpret_cast<const uint2 *>(X + ((size_t)yy * w + xx) * kC + ci);
+            const s16x4 a = __builtin_bit_cast(s16x4, v);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[kb][0], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[kb][1], acc1, 0, 0, 0);
+        }
+        // lane holds D[pixel 4*lq + i][channel lr] of each 16-channel tile
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+            const f32x4 acc = nt ? acc1 : acc0;
+            const int ch = 16 * nt + lr;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int x = x0 + 4 * lq + i;
+                if (x >= w)
+                    continue;
+                const size_t o = ((size_t)y * w + x) * kC + ch;
+                float v = acc[i];
+                if (mode == 0) {
+                    v = fmaxf(v + bias[ch], 0.f);
+                    reinterpret_cast<uint16_t *>(Y)[o] = to_bf16(v);
+                } else if (mode == 1) {
+                    reinterpret_cast<float *>(Y)[o] = v + bias[ch];
+                } else {
+                    v = from_bf16(mask[o]) > 0.f ? v : 0.f;
+                    reinterpret_cast<uint16_t *>(Y)[o] = to_bf16(v);
+                }
+            }
+        }
+    }
+}
+
+// ---- weight gradient: dW[co][ci][tap] += sum_p dY[p][co] * X[p + off(tap)][ci] ----
+// One wave per (row band, kernel row ky): K = pixels in steps of 16 through
+// LDS tiles stored channel-major, so each MFMA operand is one 8-byte LDS read.
+constexpr int kBandRows = 2;
+__global__ __launch_bounds__(64) void k_conv_wgrad(const uint16_t *__restrict__ X, const uint16_t *__restrict__ dY,
+                                                   int w, int h, float *__restrict__ dW)
+{
+    __shared__ uint16_t sdy[kC][16];
+    __shared__ uint16_t sx[kC][20]; // 16 pixels + halo, padded
+    const int l = threadIdx.x, lr = l & 15, lq = l >> 4;
+    const int ky = blockIdx.y;
+    const int y0 = blockIdx.x * kBandRows;
+    f32x4 acc[3][2][2];
+#pragma unroll
+    for (int kx = 0; kx < 3; kx++)
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+                acc[kx][mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int y = y0; y < min(y0 + kBandRows, h); y++) {
+        const int ys = y + ky - 1;
+        for (int x0 = 0; x0 < w; x0 += 16) {
+            // stage dY[y][x0..x0+15][:] and X[ys][x0-1..x0+16][:] channel-major
+            for (int i = l; i < 16 * kC; i += 64) {
+                const int px = i / kC, c = i % kC, x = x0 + px;
+                sdy[c][px] = x < w ? dY[((size_t)y * w + x) * kC + c] : (uint16_t)0;
+            }
+            for (int i = l; i < 18 * kC; i += 64) {
+                const int px = i / kC, c = i % kC, x = x0 - 1 + px;
+                sx[c][px] = (x >= 0 && x < w && ys >= 0 && ys < h) ? X[((size_t)ys * w + x) * kC + c]
+                                                                  : (uint16_t)0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++) {
+                s16x4 a;
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    a[i] = (short)sdy[16 * mt + lr][4 * lq + i];
+#pragma unroll
+                for (int kx = 0; kx < 3; kx++)
+#pragma unroll
+                    for (int nt = 0; nt < 2; nt++) {
+                        s16x4 b;
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            b[i] = (short)sx[16 * nt + lr][4 * lq + i + kx];
+                        acc[kx][mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[kx][mt][nt], 0, 0, 0);
+                    }
+            }
+            __syncthreads();
+        }
+    }
+    // D[co = 16mt + 4lq + i][ci = 16nt + lr]
+#pragma unroll
+    for (int kx = 0; kx < 3; kx++)
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int co = 16 * mt + 4 * lq + i, ci = 16 * nt + lr;
+                    atomicAdd(&dW[(co * kC + ci) * 9 + 3 * ky + kx], acc[kx][mt][nt][i]);
+                }
+}
+
+// db[c] += sum over a band of pixels of dY[p][c]
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ dY, int npix, float *__restrict__ db)
+{
+    const int c = threadIdx.x & 31, sub = threadIdx.x >> 5;
+    float s = 0.f;
+    for (int p = blockIdx.x * 8 + sub; p < npix; p += gridDim.x * 8)
+        s += from_bf16(dY[(size_t)p * kC + c]);
+    atomicAdd(&db[c], s);
+}
+
+// ---- tile head ----
+// hist[t][k]: counts of target disparities of tile t in bin k
+__global__ __launch_bounds__(256) void k_tile_hist(const uint8_t *__restrict__ disp, int W, int H, int *__restrict__ hist)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= W * H)
+        return;
+    const int y = i / W, x = i - y * W, tx = (W + 15) / 16;
+    atomicAdd(&hist[((y >> 4) * tx + (x >> 4)) * kC + (disp[i] >> 2)], 1);
+}
+
+// One 32-lane half-wave per tile (lane = bin).  Inference writes blk / conf;
+// training (hist != nullptr) also writes dlogits for the tile's pixels and
+// adds the tile's cross-entropy / T to *loss.
+__global__ __launch_bounds__(64) void k_tile_head(const float *__restrict__ logits, int w, int h,
+                                                  const int *__restrict__ hist, float thresh,
+                                                  uint8_t *__restrict__ blk, float *__restrict__ conf,
+                                                  uint16_t *__restrict__ dlogits, float *__restrict__ loss)
+{
+    const int k = threadIdx.x & 31;
+    const int tw = (w + 3) / 4, th = (h + 3) / 4, T = tw * th;
+    const int t = blockIdx.x * 2 + (threadIdx.x >> 5);
+    if (t >= T)
+        return;
+    const int ty = t / tw, tx = t - ty * tw;
+    float z = 0.f;
+    int n = 0;
+    for (int y = 4 * ty; y < min(4 * ty + 4, h); y++)
+        for (int x = 4 * tx; x < min(4 * tx + 4, w); x++) {
+            z += logits[((size_t)y * w + x) * kC + k];
+            n++;
+        }
+    z /= (float)n;
+    float mx = z;
+    for (int o = 16; o; o >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, o, 32));
+    const float e = expf(z - mx);
+    float se = e;
+    for (int o = 16; o; o >>= 1)
+        se += __shfl_xor(se, o, 32);
+    const float p = e / se;
+    // probability mass of the window of bins [4 b0, 4 b0 + 8)
+    float best = -1.f;
+    int bb = 0;
+    for (int b0 = 0; b0 < kDisp / 16 - 1; b0++) {
+        float m = (k >= 4 * b0 && k < 4 * b0 + 8) ? p : 0.f;
+        for (int o = 16; o; o >>= 1)
+            m += __shfl_xor(m, o, 32);
+        if (m > best) {
+            best = m;
+            bb = b0;
+        }
+    }
+    if (k == 0) {
+        conf[t] = best;
+        blk[t] = best >= thresh ? (uint8_t)(bb | ((bb + 1) << 4)) : (uint8_t)((kDisp / 16 - 1) << 4);
+    }
+    if (hist) {
+        int c = hist[t * kC + k], tot = c;
+        for (int o = 16; o; o >>= 1)
+            tot += __shfl_xor(tot, o, 32);
+        const float q = (float)c / (float)tot;
+        float ce = -q * logf(fmaxf(p, 1e-30f));
+        for (int o = 16; o; o >>= 1)
+            ce += __shfl_xor(ce, o, 32);
+        if (k == 0)
+            atomicAdd(loss, ce / (float)T);
+        const uint16_t g = to_bf16((p - q) / (float)T / (float)n);
+        for (int y = 4 * ty; y < min(4 * ty + 4, h); y++)
+            for (int x = 4 * tx; x < min(4 * tx + 4, w); x++)
+                dlogits[((size_t)y * w + x) * kC + k] = g;
+    }
+}
+
+// ---- parameters ----
+// forward pack: Wf[co][tap*32 + ci] = W[co][ci][tap]; backward-data pack
+// (conv of dY with flipped taps, channels swapped): Wb[ci][tap*32 + co] = W[co][ci][8 - tap]
+__global__ __launch_bounds__(256) void k_pack(const float *__restrict__ theta, uint16_t *__restrict__ Wf,
+                                              uint16_t *__restrict__ Wb)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= kLayers * kWPer)
+        return;
+    const int layer = i / kWPer, r = i - layer * kWPer;
+    const int co = r / (kC * 9), ci = (r / 9) % kC, tap = r % 9;
+    const uint16_t v = to_bf16(theta[layer * kPPer + r]);
+    Wf[layer * kWPer + co * 288 + tap * kC + ci] = v;
+    Wb[layer * kWPer + ci * 288 + (8 - tap) * kC + co] = v;
+}
+
+// Adam on fp32 master parameters (grad layout = parameter layout)
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ theta, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v, int n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi, vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    theta[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ __launch_bounds__(256) void k_zero(float *p, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        p[i] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_zero_i(int *p, int n)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+} // namespace
+
+struct sgmh_cnn {
+    sgmh_ctx *ctx = nullptr;
+    float *theta = nullptr, *grad = nullptr, *m = nullptr, *v = nullptr;
+    uint16_t *Wf = nullptr, *Wb = nullptr;
+    int step = 0;
+    bool packed = false;
+    // activations for the current frame size
+    int W = 0, H = 0;
+    int32_t *Ld = nullptr, *Rd = nullptr;
+    uint16_t *act[kLayers] = {};   // act[i] = input of layer i (bf16 NHWC)
+    float *logits = nullptr;
+    uint16_t *g0 = nullptr, *g1 = nullptr; // gradient ping-pong (bf16)
+    int *hist = nullptr;
+    float *loss = nullptr;
+    uint8_t *blk = nullptr;
+    float *conf = nullptr;
+};
+
+namespace {
+
+int cnn_resize(sgmh_cnn *c, int W, int H)
+{
+    if (c->W == W && c->H == H)
+        return 0;
+    CK(hipDeviceSynchronize());
+    void **bufs[] = {(void **)&c->Ld, (void **)&c->Rd, (void **)&c->logits, (void **)&c->g0, (void **)&c->g1,
+                     (void **)&c->hist, (void **)&c->loss, (void **)&c->blk, (void **)&c->conf};
+    for (void **b : bufs)
+        if (*b) {
+            CK(hipFree(*b));
+            *b = nullptr;
+        }
+    for (auto &a : c->act)
+        if (a) {
+            CK(hipFree(a));
+            a = nullptr;
+        }
+    const size_t w = W / 4, h = H / 4, np = w * h, T = ((w + 3) / 4) * ((h + 3) / 4);
+    CK(hipMalloc(&c->Ld, np * 4));
+    CK(hipMalloc(&c->Rd, np * 4));
+    for (auto &a : c->act)
+        CK(hipMalloc(&a, np * kC * 2));
+    CK(hipMalloc(&c->logits, np * kC * 4));
+    CK(hipMalloc(&c->g0, np * kC * 2));
+    CK(hipMalloc(&c->g1, np * kC * 2));
+    CK(hipMalloc(&c->hist, T * kC * 4));
+    CK(hipMalloc(&c->loss, 64));
+    CK(hipMalloc(&c->blk, T));
+    CK(hipMalloc(&c->conf, T * 4));
+    c->W = W;
+    c->H = H;
+    return 0;
+}
+
+int cnn_check(sgmh_cnn *c, int W, int H)
+{
+    REQUIRE(c, "sgmh_cnn: NULL network");
+    REQUIRE(W >= 16 && H >= 16 && W % 4 == 0 && H % 4 == 0 && W <= 16384 && H <= 16384,
+            "sgmh_cnn: frame %dx%d (W and H must be multiples of 4, >= 16)", W, H);
+    CK(hipSetDevice(c->ctx->device));
+    return cnn_resize(c, W, H);
+}
+
+int cnn_pack(sgmh_cnn *c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pack, dim3((kLayers * kWPer + 255) / 256), dim3(256), 0, s, c->theta, c->Wf, c->Wb);
+    CK(hipGetLastError());
+    c->packed = true;
+    return 0;
+}
+
+int conv_blocks(int w, int h) { return std::min(((w + 15) / 16) * h, 2048); }
+
+// features + 6 layers; logits in c->logits
+int cnn_forward(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, hipStream_t s)
+{
+    const int w = W / 4, h = H / 4, np = w * h;
+    if (!c->packed && cnn_pack(c, s))
+        return -1;
+    hipLaunchKernelGGL(k_blocksum, dim3((np + 255) / 256), dim3(256), 0, s, l, W, H, c->Ld);
+    hipLaunchKernelGGL(k_blocksum, dim3((np + 255) / 256), dim3(256), 0, s, r, W, H, c->Rd);
+    hipLaunchKernelGGL(k_features, dim3((np * kC + 255) / 256), dim3(256), 0, s, c->Ld, c->Rd, w, h, c->act[0]);
+    for (int i = 0; i < kLayers; i++) {
+        const bool last = i == kLayers - 1;
+        hipLaunchKernelGGL(k_conv, dim3(conv_blocks(w, h)), dim3(64), 0, s, c->act[i], w, h, c->Wf + i * kWPer,
+                           c->theta + i * kPPer + kWPer, last ? 1 : 0, (const uint16_t *)nullptr,
+                           last ? (void *)c->logits : (void *)c->act[i + 1]);
+    }
+    CK(hipGetLastError());
+    return 0;
+}
+
+// backward from dlogits (in c->g0) into c->grad (zeroed here)
+int cnn_backward(sgmh_cnn *c, int W, int H, hipStream_t s)
+{
+    const int w = W / 4, h = H / 4, np = w * h;
+    hipLaunchKernelGGL(k_zero, dim3((kNParam + 255) / 256), dim3(256), 0, s, c->grad, kNParam);
+    uint16_t *dy = c->g0, *dx = c->g1;
+    for (int i = kLayers - 1; i >= 0; i--) {
+        float *gw = c->grad + i * kPPer, *gb = gw + kWPer;
+        hipLaunchKernelGGL(k_conv_wgrad, dim3((h + kBandRows - 1) / kBandRows, 3), dim3(64), 0, s, c->act[i], dy,
+                           w, h, gw);
+        hipLaunchKernelGGL(k_bias_grad, dim3(std::min((np + 7) / 8, 1024)), dim3(256), 0, s, dy, np, gb);
+        if (i > 0) {
+            // dX = conv(dY, flipped / transposed W) * (act[i] > 0)
+            hipLaunchKernelGGL(k_conv, dim3(conv_blocks(w, h)), dim3(64), 0, s, dy, w, h, c->Wb + i * kWPer,
+                               (const float *)nullptr, 2, c->act[i], (void *)dx);
+            std::swap(dy, dx);
+        }
+    }
+    CK(hipGetLastError());
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+size_t sgmh_cnn_num_params(void) { return (size_t)kNParam; }
+
+int sgmh_cnn_create(sgmh_ctx *ctx, sgmh_cnn **out)
+{
+    REQUIRE(ctx && out, "sgmh_cnn_create: NULL argument");
+    *out = nullptr;
+    CK(hipSetDevice(ctx->device));
+    sgmh_cnn *c = new sgmh_cnn();
+    c->ctx = ctx;
+    const size_t nb = (size_t)kNParam * 4;
+    if (hipMalloc(&c->theta, nb) != hipSuccess || hipMalloc(&c->grad, nb) != hipSuccess ||
+        hipMalloc(&c->m, nb) != hipSuccess || hipMalloc(&c->v, nb) != hipSuccess ||
+        hipMalloc(&c->Wf, (size_t)kLayers * kWPer * 2) != hipSuccess ||
+        hipMalloc(&c->Wb, (size_t)kLayers * kWPer * 2) != hipSuccess) {
+        set_err("sgmh_cnn_create: out of device memory");
+        sgmh_cnn_destroy(c);
+        return -1;
+    }
+    // zero parameters and optimiser state: callers set weights before use
+    std::vector<float> z(kNParam, 0.f);
+    if (hipMemcpy(c->theta, z.data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->m, z.data(), nb, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->v, z.data(), nb, hipMemcpyHostToDevice) != hipSuccess) {
+        set_err("sgmh_cnn_create: initialisation copy failed");
+        sgmh_cnn_destroy(c);
+        return -1;
+    }
+    *out = c;
+    return 0;
+}
+
+void sgmh_cnn_destroy(sgmh_cnn *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->ctx->device);
+    (void)hipDeviceSynchronize();
+    void *bufs[] = {c->theta, c->grad, c->m, c->v, c->Wf, c->Wb, c->Ld, c->Rd, c->logits, c->g0, c->g1,
+                    c->hist, c->loss, c->blk, c->conf};
+    for (void *b : bufs)
+        if (b)
+            (void)hipFree(b);
+    for (auto a : c->act)
+        if (a)
+            (void)hipFree(a);
+    delete c;
+}
+
+int sgmh_cnn_set_params(sgmh_cnn *c, const float *theta, size_t n)
+{
+    REQUIRE(c && theta && n == (size_t)kNParam, "sgmh_cnn_set_params: need %d parameters", kNParam);
+    CK(hipSetDevice(c->ctx->device));
+    CK(hipMemcpy(c->theta, theta, n * 4, hipMemcpyHostToDevice));
+    std::vector<float> z(kNParam, 0.f);
+    CK(hipMemcpy(c->m, z.data(), n * 4, hipMemcpyHostToDevice));
+    CK(hipMemcpy(c->v, z.data(), n * 4, hipMemcpyHostToDevice));
+    c->step = 0;
+    c->packed = false;
+    return 0;
+}
+
+int sgmh_cnn_get_params(sgmh_cnn *c, float *theta, size_t n)
+{
+    REQUIRE(c && theta && n == (size_t)kNParam, "sgmh_cnn_get_params: need %d parameters", kNParam);
+    CK(hipSetDevice(c->ctx->device));
+    CK(hipDeviceSynchronize());
+    CK(hipMemcpy(theta, c->theta, n * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int sgmh_cnn_propose_dev(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, float thresh,
+                         uint8_t *d_blk, float *d_conf, void *stream)
+{
+    REQUIRE(l && r && d_blk, "sgmh_cnn_propose_dev: NULL argument");
+    if (cnn_check(c, W, H))
+        return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (cnn_forward(c, l, r, W, H, s))
+        return -1;
+    const int w = W / 4, h = H / 4, T = ((w + 3) / 4) * ((h + 3) / 4);
+    hipLaunchKernelGGL(k_tile_head, dim3((T + 1) / 2), dim3(64), 0, s, c->logits, w, h, (const int *)nullptr,
+                       thresh, d_blk, d_conf ? d_conf : c->conf, (uint16_t *)nullptr, (float *)nullptr);
+    CK(hipGetLastError());
+    return 0;
+}
+
+// One optimisation step on one frame: forward, tile cross-entropy against the
+// target disparities, backward, Adam.  *loss (host) receives the loss before
+// the update (this synchronises the stream).  lr <= 0: gradients only, kept in
+// the network's gradient buffer (sgmh_cnn_get_grads), no update.
+int sgmh_cnn_train_step_dev(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, const uint8_t *target, int W,
+                            int H, float lr, float *loss, void *stream)
+{
+    REQUIRE(l && r && target, "sgmh_cnn_train_step_dev: NULL argument");
+    if (cnn_check(c, W, H))
+        return -1;
+    hipStream_t s = (hipStream_t)stream;
+    if (cnn_forward(c, l, r, W, H, s))
+        return -1;
+    const int w = W / 4, h = H / 4, T = ((w + 3) / 4) * ((h + 3) / 4);
+    hipLaunchKernelGGL(k_zero_i, dim3((T * kC + 255) / 256), dim3(256), 0, s, c->hist, T * kC);
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, s, c->loss, 1);
+    hipLaunchKernelGGL(k_tile_hist, dim3((W * H + 255) / 256), dim3(256), 0, s, target, W, H, c->hist);
+    hipLaunchKernelGGL(k_tile_head, dim3((T + 1) / 2), dim3(64), 0, s, c->logits, w, h, c->hist, 2.f, c->blk,
+                       c->conf, c->g0, c->loss);
+    if (cnn_backward(c, W, H, s))
+        return -1;
+    if (lr > 0.f) {
+        c->step++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        hipLaunchKernelGGL(k_adam, dim3((kNParam + 255) / 256), dim3(256), 0, s, c->theta, c->grad, c->m, c->v,
+                           kNParam, lr, b1, b2, 1e-8f, 1.f - powf(b1, (float)c->step),
+                           1.f - powf(b2, (float)c->step));
+        if (cnn_pack(c, s))
+            return -1;
+    }
+    CK(hipGetLastError());
+    if (loss) {
+        CK(hipMemcpyAsync(loss, c->loss, 4, hipMemcpyDeviceToHost, s));
+        CK(hipStreamSynchronize(s));
+    }
+    return 0;
+}
+
+int sgmh_cnn_get_grads(sgmh_cnn *c, float *g, size_t n)
+{
+    REQUIRE(c && g && n == (size_t)kNParam, "sgmh_cnn_get_grads: need %d values", kNParam);
+    CK(hipSetDevice(c->ctx->device));
+    CK(hipDeviceSynchronize());
+    CK(hipMemcpy(g, c->grad, n * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Features (layer-0 input, bf16 bits) and logits of the last forward pass.
+int sgmh_cnn_get_activations(sgmh_cnn *c, uint16_t *features, float *logits)
+{
+    REQUIRE(c && c->W > 0, "sgmh_cnn_get_activations: no forward pass yet");
+    CK(hipSetDevice(c->ctx->device));
+    CK(hipDeviceSynchronize());
+    const size_t np = (size_t)(c->W / 4) * (c->H / 4);
+    if (features)
+        CK(hipMemcpy(features, c->act[0], np * kC * 2, hipMemcpyDeviceToHost));
+    if (logits)
+        CK(hipMemcpy(logits, c->logits, np * kC * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+} // extern "C"

@@ -644,4 +644,137 @@ __device__ __forceinline__ void wta_batch(const uint8_t *__restrict__ L, size_t 
 
 constexpr int kWtaBatch = 2; // pixel batches per wta_batch call
 
+// ---------------------------------------------------------------------------
+// windowed SGM (build-defined; no reference counterpart)
+// ---------------------------------------------------------------------------
+// Each 16x16 tile t has an active range of 16-disparity blocks
+// [blk[t] & 15, blk[t] >> 4]; the recurrence sees only the predecessor's
+// active disparities (the rest are absent, +inf), mn is the minimum over the
+// predecessor's active range, and m(d) saturates at 255 as in the reference
+// step.  With every block active it is the plain path exactly.
+//
+// Layout: lane l of a 16-lane row holds d = 16*b + l in register b, so a
+// window of k blocks keeps every lane busy with k registers, and a window that
+// moves at a tile boundary needs no lane realignment.  d-1 / d+1 are a row
+// rotation by one lane, with lane 0 / lane 15 taking the neighbouring block's
+// value.  The four scanlines of a wave share one tile (groups of 4 rows or 4
+// columns never straddle a 16-pixel tile), so the active-block test is a
+// scalar branch.
+constexpr uint32_t kWinAbsent = 0x3FFu;
+constexpr int kRowRor1 = 0x121;  // lane i <- lane (i-1) mod 16
+constexpr int kRowRor15 = 0x12F; // lane i <- lane (i+1) mod 16
+
+struct WinArgs {
+    const uint64_t *cl, *cr; // census codes, pixel i at index kLead + i
+    const uint8_t *blk;      // per-tile block ranges
+    int W, H, D, tiles_x;
+    uint32_t out_bytes;
+    int ndirs;
+    DirTask dir[4];
+};
+
+template <int NB, bool HORIZ>
+__device__ __forceinline__ void path_window_group(const WinArgs &a, const DirTask &t, int grp)
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+    const int W = a.W, H = a.H, D = a.D, dx = t.dx, dy = t.dy;
+    const int line = grp * kLinesPerWave + g;
+    const uint32_t live = line < t.nlines ? 0u : kDrop;
+    const int ln = line < t.nlines ? line : t.nlines - 1;
+    int x = HORIZ ? (dx > 0 ? 0 : W - 1) : ln;
+    int y = HORIZ ? ln : (dy > 0 ? 0 : H - 1);
+    const int steps = HORIZ ? W : H;
+    // wave-uniform tile coordinate along the fixed axis
+    const int fixed_tile = (grp * kLinesPerWave) >> 4;
+    const uint32_t nb8 = (uint32_t)(W * H + kLead) * 8u;
+    const rsrc_t rl = make_rsrc(a.cl, nb8), rr = make_rsrc(a.cr, nb8);
+    const rsrc_t ro = make_rsrc(t.L, a.out_bytes);
+    const uint32_t P1 = (uint32_t)t.P1, P2 = kPlainP2, inval = kPlainInvalid;
+    const bool lane0 = l == 0, lane15 = l == 15;
+
+    uint32_t L[NB], mn = 0;
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+        L[b] = 0; // a virtual predecessor with all blocks at 0: the first step gives L = C
+    for (int s = 0; s < steps; s++) {
+        const int tx = HORIZ ? (x >> 4) : fixed_tile, ty = HORIZ ? fixed_tile : (y >> 4);
+        const uint32_t rng = __builtin_amdgcn_readfirstlane((uint32_t)a.blk[ty * a.tiles_x + tx]);
+        const int b1 = min((int)(rng >> 4), NB - 1), b0 = min((int)(rng & 15u), b1); // clamped: a bad table cannot index past the pixel
+        const int pix = y * W + x;
+        const uint64_t cL = ld64(rl, (uint32_t)(pix + kLead) * 8u);
+        uint64_t cR[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+            if (b >= b0 && b <= b1)
+                cR[b] = ld64(rr, (uint32_t)(pix + kLead - 16 * b - l) * 8u);
+        uint32_t aP[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+            aP[b] = L[b] + P1;
+        const uint32_t jump = mn + P2;
+        uint32_t nL[NB], lo = 0xFFFFFFFFu;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            if (b >= b0 && b <= b1) {
+                const uint32_t h = (uint32_t)__popcll(cL ^ cR[b]);
+                const uint32_t C = x - 16 * b - l >= 0 ? h : inval;
+                const uint32_t r1 = dpp<kRowRor1>(aP[b]);
+                const uint32_t r1m = b > 0 ? dpp<kRowRor1>(aP[b - 1]) : 0xFFFFFFFFu;
+                const uint32_t r15 = dpp<kRowRor15>(aP[b]);
+                const uint32_t r15p = b < NB - 1 ? dpp<kRowRor15>(aP[b + 1]) : 0xFFFFFFFFu;
+                const uint32_t below = lane0 ? r1m : r1;
+                const uint32_t above = lane15 ? r15p : r15;
+                const uint32_t m = umin(umin(umin(L[b], jump), umin(below, above)), 255u);
+                nL[b] = C + m - mn;
+                lo = umin(lo, nL[b]);
+            } else {
+                nL[b] = kWinAbsent;
+            }
+        }
+        mn = row_min16(lo);
+        const uint32_t base = (uint32_t)pix * (uint32_t)D + (uint32_t)l;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            L[b] = nL[b];
+            if (b >= b0 && b <= b1)
+                __builtin_amdgcn_raw_buffer_store_b8((uint8_t)nL[b], ro, (base + 16u * b) | live, 0, 0);
+        }
+        x += dx;
+        y += dy;
+    }
+}
+
+// Sum over the directions and argmin over the pixel's active range; one
+// thread per pixel, 16 disparities per load.
+template <int NB>
+__device__ __forceinline__ void wta_window_px(const uint8_t *__restrict__ L, size_t dir_stride,
+                                              int ndirs, const uint8_t *__restrict__ blk, int tiles_x,
+                                              int W, int p, uint8_t *__restrict__ disp)
+{
+    const int y = p / W, x = p - y * W;
+    const uint32_t rng = blk[(y >> 4) * tiles_x + (x >> 4)];
+    const int b1 = min((int)(rng >> 4), NB - 1), b0 = min((int)(rng & 15u), b1);
+    uint32_t best = 0xFFFFFFFFu;
+    for (int b = b0; b <= b1; b++) {
+        uint32_t ev[4] = {0, 0, 0, 0}, od[4] = {0, 0, 0, 0};
+        for (int k = 0; k < ndirs; k++) {
+            const uint4 q = *reinterpret_cast<const uint4 *>(L + k * dir_stride + (size_t)p * (16 * NB) + 16 * b);
+            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                ev[i] += w[i] & 0x00FF00FFu;
+                od[i] += (w[i] >> 8) & 0x00FF00FFu;
+            }
+        }
+        const uint32_t d0 = 16u * b;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t db = d0 + 4 * i;
+            best = umin(best, umin(((ev[i] & 0xFFFFu) << 8) | db, ((od[i] & 0xFFFFu) << 8) | (db + 1)));
+            best = umin(best, umin(((ev[i] >> 16) << 8) | (db + 2), ((od[i] >> 16) << 8) | (db + 3)));
+        }
+    }
+    disp[p] = (uint8_t)(best & 0xFFu);
+}
+
 } // namespace sgmk

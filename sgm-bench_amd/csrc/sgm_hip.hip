@@ -113,6 +113,31 @@ __global__ __launch_bounds__(256) void k_wta(const uint8_t *__restrict__ L, size
         wta_batch<LPP, NDIRS>(L, dir_stride, npix, disp, b0);
 }
 
+// windowed SGM: every direction of the frame in one launch, one line group
+// (4 rows or 4 columns) per wave
+template <int NB>
+__global__ __launch_bounds__(64) void k_path_window(WinArgs a)
+{
+    int k = 0;
+    while (k + 1 < a.ndirs && (int)blockIdx.x >= a.dir[k + 1].first_block)
+        k++;
+    const DirTask t = a.dir[k];
+    const int grp = (int)blockIdx.x - t.first_block;
+    if (t.dy == 0)
+        path_window_group<NB, true>(a, t, grp);
+    else
+        path_window_group<NB, false>(a, t, grp);
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_wta_window(const uint8_t *__restrict__ L, size_t dir_stride, int ndirs,
+                                                    const uint8_t *__restrict__ blk, int tiles_x, int W, int npix,
+                                                    uint8_t *__restrict__ disp)
+{
+    for (int p = blockIdx.x * 256 + threadIdx.x; p < npix; p += gridDim.x * 256)
+        wta_window_px<NB>(L, dir_stride, ndirs, blk, tiles_x, W, p, disp);
+}
+
 // ---------------------------------------------------------------------------
 // streaming-copy probe: the achievable HBM ceiling and the reference size for
 // scaling hardware byte counters (the pattern of the reference's
@@ -295,6 +320,64 @@ static int plain_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const u
         PhaseScope ps(ctx, s, PH_ARGMIN);
         if (launch_wta(s, vol, N * D, paths, N, D, disp))
             return -1;
+    }
+    return 0;
+}
+
+static int window_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W, int H,
+                          int D, const uint8_t *blk, uint8_t *disp)
+{
+    const size_t N = (size_t)W * H;
+    REQUIRE(N * D < (1ull << 31), "sgmh_window: %dx%d D=%d exceeds the 2 GiB per-volume limit", W, H, D);
+    if (ensure(ctx->cl, (N + kLead) * 8) || ensure(ctx->cr, (N + kLead) * 8) || ensure(ctx->vol, N * D * 4))
+        return -1;
+    uint64_t *cl = (uint64_t *)ctx->cl.p, *cr = (uint64_t *)ctx->cr.p;
+    uint8_t *vol = (uint8_t *)ctx->vol.p;
+    {
+        PhaseScope ps(ctx, s, PH_CENSUS);
+        if (launch_census(s, l, cl + kLead, W, H, W) || launch_census(s, r, cr + kLead, W, H, W))
+            return -1;
+    }
+    WinArgs a{};
+    a.cl = cl;
+    a.cr = cr;
+    a.blk = blk;
+    a.W = W;
+    a.H = H;
+    a.D = D;
+    a.tiles_x = (W + 15) / 16;
+    a.out_bytes = (uint32_t)(N * D);
+    a.ndirs = 4;
+    int blocks = 0;
+    for (int k = 0; k < 4; k++) {
+        DirTask &t = a.dir[k];
+        t.dx = kDirsPlain[k][0];
+        t.dy = kDirsPlain[k][1];
+        t.P1 = (int)kPlainP1;
+        t.nlines = t.dy == 0 ? H : W;
+        t.first_block = blocks;
+        t.L = vol + (size_t)k * N * D;
+        blocks += (t.nlines + kLinesPerWave - 1) / kLinesPerWave;
+    }
+    {
+        PhaseScope ps(ctx, s, PH_AGG);
+        switch (D / 16) {
+        case 2: hipLaunchKernelGGL(k_path_window<2>, dim3(blocks), dim3(64), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_path_window<4>, dim3(blocks), dim3(64), 0, s, a); break;
+        case 8: hipLaunchKernelGGL(k_path_window<8>, dim3(blocks), dim3(64), 0, s, a); break;
+        default: set_err("sgmh_window: D=%d (32, 64 or 128)", D); return -1;
+        }
+        CK(hipGetLastError());
+    }
+    {
+        PhaseScope ps(ctx, s, PH_ARGMIN);
+        const int wb = (int)std::min<size_t>((N + 255) / 256, 16384);
+        switch (D / 16) {
+        case 2: hipLaunchKernelGGL(k_wta_window<2>, dim3(wb), dim3(256), 0, s, vol, N * D, 4, blk, a.tiles_x, W, (int)N, disp); break;
+        case 4: hipLaunchKernelGGL(k_wta_window<4>, dim3(wb), dim3(256), 0, s, vol, N * D, 4, blk, a.tiles_x, W, (int)N, disp); break;
+        default: hipLaunchKernelGGL(k_wta_window<8>, dim3(wb), dim3(256), 0, s, vol, N * D, 4, blk, a.tiles_x, W, (int)N, disp); break;
+        }
+        CK(hipGetLastError());
     }
     return 0;
 }
@@ -592,6 +675,56 @@ int sgmh_path_plain_dev(sgmh_ctx *c, const uint64_t *cl, const uint64_t *cr, int
     CK(hipMemcpyAsync(pr + kLead, cr, N * 8, hipMemcpyDeviceToDevice, s));
     const int dirs[1][3] = {{dx, dy, (int)kPlainP1}};
     return plain_paths(c, s, pl, pr, W, H, D, dirs, 1, L, N * D);
+}
+
+int sgmh_window_dev(sgmh_ctx *c, const uint8_t *l, const uint8_t *r, int W, int H, int D,
+                    const uint8_t *blk, uint8_t *disp, void *stream)
+{
+    REQUIRE(c && l && r && blk && disp, "sgmh_window_dev: NULL argument");
+    if (check_dims(W, H, D))
+        return -1;
+    REQUIRE(D == 32 || D == 64 || D == 128, "sgmh_window_dev: D=%d (32, 64 or 128)", D);
+    CK(hipSetDevice(c->device));
+    hipStream_t s = pick(c, stream);
+    begin_call(c, s);
+    return window_enqueue(c, s, l, r, W, H, D, blk, disp);
+}
+
+int sgmh_window(sgmh_ctx *c, const uint8_t *left, const uint8_t *right, int W, int H, int D,
+                const uint8_t *blk, uint8_t *disp, sgmh_times *t)
+{
+    REQUIRE(c && left && right && blk && disp, "sgmh_window: NULL argument");
+    if (check_dims(W, H, D))
+        return -1;
+    REQUIRE(D == 32 || D == 64 || D == 128, "sgmh_window: D=%d (32, 64 or 128)", D);
+    const int ntiles = ((W + 15) / 16) * ((H + 15) / 16);
+    for (int i = 0; i < ntiles; i++)
+        REQUIRE((blk[i] & 15) <= (blk[i] >> 4) && (blk[i] >> 4) < D / 16,
+                "sgmh_window: tile %d has block range %d..%d outside 0..%d", i, blk[i] & 15, blk[i] >> 4,
+                D / 16 - 1);
+    CK(hipSetDevice(c->device));
+    const size_t N = (size_t)W * H;
+    if (ensure(c->img_l, N) || ensure(c->img_r, N) || ensure(c->disp, N) || ensure(c->disp1, (size_t)ntiles))
+        return -1;
+    hipStream_t s = c->stream;
+    begin_call(c, s);
+    {
+        PhaseScope ps(c, s, PH_XFER);
+        CK(hipMemcpyAsync(c->img_l.p, left, N, hipMemcpyHostToDevice, s));
+        CK(hipMemcpyAsync(c->img_r.p, right, N, hipMemcpyHostToDevice, s));
+        CK(hipMemcpyAsync(c->disp1.p, blk, (size_t)ntiles, hipMemcpyHostToDevice, s));
+    }
+    if (window_enqueue(c, s, (const uint8_t *)c->img_l.p, (const uint8_t *)c->img_r.p, W, H, D,
+                       (const uint8_t *)c->disp1.p, (uint8_t *)c->disp.p))
+        return -1;
+    {
+        PhaseScope ps(c, s, PH_XFER);
+        CK(hipMemcpyAsync(disp, c->disp.p, N, hipMemcpyDeviceToHost, s));
+    }
+    CK(hipStreamSynchronize(s));
+    if (t)
+        return sgmh_get_times(c, t);
+    return 0;
 }
 
 int sgmh_copy_probe(sgmh_ctx *c, void *dst, const void *src, size_t bytes, void *stream)

@@ -105,6 +105,19 @@ def hip_lib() -> ctypes.CDLL:
         h.sgmh_half_dev.argtypes = [_P, _P, _I, _I, _P, _P]
         h.sgmh_path_plain_dev.argtypes = [_P, _P, _P, _I, _I, _I, _I, _I, _P, _P]
         h.sgmh_copy_probe.argtypes = [_P, _P, _P, ctypes.c_size_t, _P]
+        h.sgmh_window.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P, ctypes.POINTER(Times)]
+        h.sgmh_window_dev.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P, _P]
+        h.sgmh_cnn_num_params.restype = ctypes.c_size_t
+        h.sgmh_cnn_create.argtypes = [_P, ctypes.POINTER(_P)]
+        h.sgmh_cnn_destroy.argtypes = [_P]
+        h.sgmh_cnn_destroy.restype = None
+        h.sgmh_cnn_set_params.argtypes = [_P, _P, ctypes.c_size_t]
+        h.sgmh_cnn_get_params.argtypes = [_P, _P, ctypes.c_size_t]
+        h.sgmh_cnn_propose_dev.argtypes = [_P, _P, _P, _I, _I, ctypes.c_float, _P, _P, _P]
+        h.sgmh_cnn_train_step_dev.argtypes = [_P, _P, _P, _P, _I, _I, ctypes.c_float,
+                                              ctypes.POINTER(ctypes.c_float), _P]
+        h.sgmh_cnn_get_grads.argtypes = [_P, _P, ctypes.c_size_t]
+        h.sgmh_cnn_get_activations.argtypes = [_P, _P, _P]
         h.sgmh_batch_plan.argtypes = [_P, ctypes.POINTER(Pair), _I, _I, ctypes.POINTER(_P)]
         h.sgmh_batch_run.argtypes = [_P, _P]
         h.sgmh_batch_run_loop.argtypes = [_P, _P]
@@ -115,7 +128,10 @@ def hip_lib() -> ctypes.CDLL:
                      "sgmh_plain_dev", "sgmh_ms", "sgmh_ms_dev", "sgmh_census_dev",
                      "sgmh_sobel_dev", "sgmh_half_dev", "sgmh_path_plain_dev", "sgmh_copy_probe",
                      "sgmh_batch_plan", "sgmh_batch_run", "sgmh_batch_run_loop",
-                     "sgmh_batch_describe"):
+                     "sgmh_batch_describe", "sgmh_window", "sgmh_window_dev",
+                     "sgmh_cnn_create", "sgmh_cnn_set_params", "sgmh_cnn_get_params",
+                     "sgmh_cnn_propose_dev", "sgmh_cnn_train_step_dev", "sgmh_cnn_get_grads",
+                     "sgmh_cnn_get_activations"):
             getattr(h, name).restype = _I
         _hip = h
     return _hip
@@ -321,6 +337,25 @@ class Context:
     def half_dev(self, d_img: int, W: int, H: int, d_out: int, stream: int = 0) -> None:
         self._ck(self._lib.sgmh_half_dev(self._h, d_img, W, H, d_out, stream or None))
 
+    def window(self, left: np.ndarray, right: np.ndarray, blk: np.ndarray, D: int = 128,
+               times: Optional[dict] = None) -> np.ndarray:
+        """Windowed SGM; blk[ty, tx] = first block | last block << 4."""
+        L, R = _u8(left), _u8(right)
+        H, W = L.shape
+        blk = _u8(blk)
+        out = np.empty((H, W), np.uint8)
+        t = Times()
+        self._ck(self._lib.sgmh_window(self._h, _ptr(L), _ptr(R), W, H, D, _ptr(blk), _ptr(out),
+                                       ctypes.byref(t)))
+        if times is not None:
+            times.update(t.as_dict())
+        return out
+
+    def window_dev(self, d_left: int, d_right: int, W: int, H: int, D: int, d_blk: int,
+                   d_disp: int, stream: int = 0) -> None:
+        self._ck(self._lib.sgmh_window_dev(self._h, d_left, d_right, W, H, D, d_blk, d_disp,
+                                           stream or None))
+
     def batch(self, pairs, flags: int = BATCH_AUTOTUNE) -> "Batch":
         return Batch(self, pairs, flags)
 
@@ -363,6 +398,64 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.sgmh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Cnn:
+    """The window-proposal network (sgmh_cnn).  Device pointers are ints."""
+
+    def __init__(self, ctx: Context, params: Optional[np.ndarray] = None):
+        self._ctx = ctx
+        self._lib = ctx._lib
+        h = _P()
+        ctx._ck(self._lib.sgmh_cnn_create(ctx.handle, ctypes.byref(h)))
+        self._h = h
+        self.nparams = int(self._lib.sgmh_cnn_num_params())
+        if params is not None:
+            self.set_params(params)
+
+    def set_params(self, theta: np.ndarray) -> None:
+        theta = np.ascontiguousarray(theta, np.float32)
+        assert theta.size == self.nparams
+        self._ctx._ck(self._lib.sgmh_cnn_set_params(self._h, _ptr(theta), theta.size))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.nparams, np.float32)
+        self._ctx._ck(self._lib.sgmh_cnn_get_params(self._h, _ptr(out), out.size))
+        return out
+
+    def grads(self) -> np.ndarray:
+        out = np.empty(self.nparams, np.float32)
+        self._ctx._ck(self._lib.sgmh_cnn_get_grads(self._h, _ptr(out), out.size))
+        return out
+
+    def activations(self, W: int, H: int):
+        f = np.empty((H // 4, W // 4, 32), np.uint16)
+        z = np.empty((H // 4, W // 4, 32), np.float32)
+        self._ctx._ck(self._lib.sgmh_cnn_get_activations(self._h, _ptr(f), _ptr(z)))
+        return (f.astype(np.uint32) << 16).view(np.float32), z
+
+    def propose_dev(self, d_left: int, d_right: int, W: int, H: int, thresh: float, d_blk: int,
+                    d_conf: int = 0, stream: int = 0) -> None:
+        self._ctx._ck(self._lib.sgmh_cnn_propose_dev(self._h, d_left, d_right, W, H, thresh, d_blk,
+                                                      d_conf or None, stream or None))
+
+    def train_step_dev(self, d_left: int, d_right: int, d_target: int, W: int, H: int, lr: float,
+                       stream: int = 0) -> float:
+        loss = ctypes.c_float(0.0)
+        self._ctx._ck(self._lib.sgmh_cnn_train_step_dev(self._h, d_left, d_right, d_target, W, H,
+                                                         lr, ctypes.byref(loss), stream or None))
+        return float(loss.value)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sgmh_cnn_destroy(self._h)
             self._h = None
 
     def __del__(self):
