@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void k_tile_hist(const uint8_t *__restrict__ d
     if (i >= W * H)
         return;
     const int y = i / W, x = i - y * W, tx = (W + 15) / 16;
-    atomicAdd(&hist[((y >> 4) * tx + (x >> 4)) * kC + (disp[i] >> 2)], 1);
+    atomicAdd(&hist[((y >> 4) * tx + (x >> 4)) * kC + min(disp[i] >> 2, kC - 1)], 1);
 }
 
 // One 32-lane half-wave per tile (lane = bin).  Inference writes blk / conf;
