@@ -89,6 +89,13 @@ BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
 def hip_lib() -> ctypes.CDLL:
     global _hip
     if _hip is None:
+        # One HIP runtime per process: when torch is installed, let it load its
+        # runtime first so the library binds to the same one (a runtime loaded
+        # after another has initialised the device finds no GPU).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         h = _load("libsgm_hip.so")
         h.sgmh_last_error.restype = ctypes.c_char_p
         h.sgmh_ctx_create.argtypes = [ctypes.POINTER(_P), _I]
