@@ -128,6 +128,64 @@ def test_path_L_matches_transition_form(dx, dy):
                                   path_bruteforce(C, dx, dy, 10, 200, True))
 
 
+def window_bruteforce(left, right, blk, D, paths=4):
+    """Windowed SGM from its definition, by transition matrices over the active
+    disparity sets: a predecessor disparity outside the predecessor tile's range
+    is absent, each candidate Lp(d') + pen(d, d') (pen 0 / P1 / P2 for |d-d'| =
+    0 / 1 / more) saturates at 255, the first pixel of a path takes L = C, and
+    the argmin runs over the pixel's own range (lowest d on ties)."""
+    H, W = left.shape
+    C = oracle.cost(oracle.census(left), oracle.census(right), D).astype(np.int64)
+    d = np.arange(D)
+    pen = np.where(d[:, None] == d[None, :], 0, np.where(np.abs(d[:, None] - d[None, :]) == 1, 20, 192))
+
+    def active(y, x):
+        b = int(blk[y // 16, x // 16])
+        return (d >= 16 * (b & 15)) & (d < 16 * ((b >> 4) + 1))
+
+    S = np.zeros((H, W, D), np.int64)
+    dirs = [(1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, 1), (1, -1), (-1, -1)][:paths]
+    for dx, dy in dirs:
+        Ld = np.zeros((H, W, D), np.int64)
+        for y in (range(H - 1, -1, -1) if dy < 0 else range(H)):
+            for x in (range(W - 1, -1, -1) if dx < 0 else range(W)):
+                a = active(y, x)
+                px, py = x - dx, y - dy
+                if 0 <= px < W and 0 <= py < H:
+                    ap = active(py, px)
+                    lp = np.where(ap, Ld[py, px], 1 << 20)
+                    mn = lp[ap].min()
+                    m = np.minimum(lp[None, :] + pen, 255).min(axis=1)
+                    Ld[y, x] = np.where(a, C[y, x] + m - mn, 0)
+                else:
+                    Ld[y, x] = np.where(a, C[y, x], 0)
+        S += Ld
+    out = np.empty((H, W), np.uint8)
+    for y in range(H):
+        for x in range(W):
+            a = active(y, x)
+            out[y, x] = np.flatnonzero(a)[np.argmin(S[y, x][a])]
+    return out
+
+
+@pytest.mark.parametrize("seed,paths", [(0, 4), (1, 4), (2, 8)])
+def test_window_oracle_matches_transition_form(seed, paths):
+    rng = np.random.default_rng(seed)
+    H, W, D = 21, 37, 64
+    L, R, _ = sgm_amd.scene_planes(W, H, 40, seed + 3, planes=(2, 4))
+    b0 = rng.integers(0, 3, (2, 3))
+    b1 = np.minimum(b0 + rng.integers(0, 3, b0.shape), 3)
+    blk = (b0 | (b1 << 4)).astype(np.uint8)
+    np.testing.assert_array_equal(oracle.sgm_window(L, R, blk, D, paths), window_bruteforce(L, R, blk, D, paths))
+
+
+def test_window_oracle_full_range_is_plain():
+    L, R = rng_image(40, 70, 11), rng_image(40, 70, 12)
+    for D, paths in ((32, 4), (64, 8)):
+        blk = np.full((3, 5), (D // 16 - 1) << 4, np.uint8)
+        np.testing.assert_array_equal(oracle.sgm_window(L, R, blk, D, paths), oracle.sgm_plain(L, R, D, paths))
+
+
 def test_argmin_lowest_d_wins_ties():
     S = np.full((1, 3, 8), 100, np.uint16)
     S[0, 0, [2, 5]] = 7
