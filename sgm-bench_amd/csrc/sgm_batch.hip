@@ -518,6 +518,13 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
         delete b;
         return -1;
     }
+    // zeroed once: the census lead slack of each plain pair must hold valid codes
+    if (hipMemset(b->arena, 0, off) != hipSuccess) {
+        set_err("sgmh_batch_plan: cannot clear the workspace");
+        (void)hipFree(b->arena);
+        delete b;
+        return -1;
+    }
     uint8_t *A = (uint8_t *)b->arena;
 
     // ---- phase work lists ----

@@ -194,7 +194,7 @@ __device__ __forceinline__ void st_bytes(rsrc_t r, uint32_t off, const uint32_t 
         uint32_t w[DPL / 4];
 #pragma unroll
         for (int k = 0; k < DPL / 4; k++)
-            w[k] = v[4 * k] | (v[4 * k + 1] << 8) | (v[4 * k + 2] << 16) | (v[4 * k + 3] << 24);
+            w[k] = pack4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
         if constexpr (DPL == 4) {
             __builtin_amdgcn_raw_buffer_store_b32(w[0], r, off, 0, 0);
         } else if constexpr (DPL == 8) {
@@ -293,18 +293,26 @@ __device__ __forceinline__ LineGeom line_geom(int i, int dx, int dy, int W, int 
     return {x, y, sx < sy ? sx : sy};
 }
 
-__device__ __forceinline__ uint64_t dpp64_shr1(uint64_t old, uint64_t v)
+// 64-bit row shift by one lane; the lane without a source (lane 0 for shr,
+// 15 for shl) reads 0 and takes `in` instead, which is 0 in every other lane:
+// the OR folds into the DPP move (v_or_b32_dpp, bound_ctrl:1), one op per half.
+__device__ __forceinline__ uint64_t dpp64_shr1(uint64_t in, uint64_t v)
 {
-    const uint32_t lo = dpp_or<kRowShr1>((uint32_t)old, (uint32_t)v);
-    const uint32_t hi = dpp_or<kRowShr1>((uint32_t)(old >> 32), (uint32_t)(v >> 32));
+    const uint32_t lo = dpp<kRowShr1>((uint32_t)v) | (uint32_t)in;
+    const uint32_t hi = dpp<kRowShr1>((uint32_t)(v >> 32)) | (uint32_t)(in >> 32);
     return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ uint64_t dpp64_shl1(uint64_t old, uint64_t v)
+__device__ __forceinline__ uint64_t dpp64_shl1(uint64_t in, uint64_t v)
 {
-    const uint32_t lo = dpp_or<kRowShl1>((uint32_t)old, (uint32_t)v);
-    const uint32_t hi = dpp_or<kRowShl1>((uint32_t)(old >> 32), (uint32_t)(v >> 32));
+    const uint32_t lo = dpp<kRowShl1>((uint32_t)v) | (uint32_t)in;
+    const uint32_t hi = dpp<kRowShl1>((uint32_t)(v >> 32)) | (uint32_t)(in >> 32);
     return ((uint64_t)hi << 32) | lo;
 }
+
+// An offset bias that keeps a lane's loads out of range (they return 0) for a
+// whole line: buffers stay below 2^31 bytes and a line advances the offset by
+// less than 2^31, so biased offsets never wrap back into range.
+constexpr uint32_t kFar = 0x80000000u;
 
 // Horizontal direction DX (+1: L->R, -1: R->L), cost from census codes.
 // Lane l keeps the window w[j] = cr[y][x - l*DPL - j] in registers; one step
@@ -342,13 +350,15 @@ __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, Li
         const int slot = DX > 0 ? (j + 1) % DPL : (j - 1 + DPL) % DPL;
         phys[slot] = ld64(rr, (uint32_t)(row + x - DX - lbase - j) * 8u);
     }
-    // prefetched cl[x_s] and the window code entering at step s
+    // prefetched cl[x_s] and the window code entering at step s; only the
+    // lane that takes the entering code loads it, the others read 0
     const int enter = DX > 0 ? 0 : -(D - 1);
+    const uint32_t nbias = l == (DX > 0 ? 0 : kRowLanes - 1) ? 0u : kFar;
     uint64_t pl[PF], pn[PF];
 #pragma unroll
     for (int k = 0; k < PF; k++) {
         pl[k] = ld64(rl, (uint32_t)(row + x + k * DX) * 8u);
-        pn[k] = ld64(rr, (uint32_t)(row + x + k * DX + enter) * 8u);
+        pn[k] = ld64(rr, (uint32_t)(row + x + k * DX + enter) * 8u + nbias);
         __builtin_amdgcn_sched_barrier(0); // slot 0 first: its wait is the shortest
     }
     uint32_t L[DPL], mn = 0;
@@ -388,7 +398,7 @@ __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, Li
             __builtin_amdgcn_sched_barrier(0);
             const int xp = x + PF * DX;
             pl[k] = ld64(rl, (uint32_t)(row + xp) * 8u);
-            pn[k] = ld64(rr, (uint32_t)(row + xp + enter) * 8u);
+            pn[k] = ld64(rr, (uint32_t)(row + xp + enter) * 8u + nbias);
             sgm_step<DPL, SAT>(L, C, mn, P1, P2);
             st_bytes<DPL>(ro, (!CHECK || s < steps) ? off : kDrop, L);
             off += ostep;
@@ -421,7 +431,10 @@ __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, Li
 // materialised uint8 cost volume, with a prefetch ring PF steps deep.  DECIM
 // stores L only at even (x, y), into a (W/2)x(H/2) layout (the multiscale
 // argmin reads S only there, sgm_ms_ref.c:194-199).
-template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, bool VERT, class A>
+// VERT: 0 any direction (cost select per step), 1 a column (x fixed: the
+// invalid disparities are fixed too, folded in by a max), 2 columns with no
+// invalid disparity at all (x >= D-1 on every line of the wave).
+template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, int VERT, class A>
 __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineGeom ln, int l,
                                              int maxsteps)
 {
@@ -501,8 +514,10 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
 #pragma unroll
                 for (int j = 0; j < DPL; j++) {
                     const uint32_t h = (uint32_t)__popcll(cL ^ rq[k][j]);
-                    if constexpr (VERT)
-                        C[j] = umin(255u, h > inv[j] ? h : inv[j]);
+                    if constexpr (VERT == 2)
+                        C[j] = h;
+                    else if constexpr (VERT == 1)
+                        C[j] = h > inv[j] ? h : inv[j];
                     else
                         C[j] = rel >= j ? h : inval;
                 }
@@ -568,10 +583,15 @@ __device__ __forceinline__ void path_group(const A &a, const DirTask &t, int grp
             return;
         }
     }
-    if (FROM_CENSUS && t.dx == 0)
-        path_generic<DPL, FROM_CENSUS, SAT, DECIM, true, A>(a, t, ln, l, maxsteps);
-    else
-        path_generic<DPL, FROM_CENSUS, SAT, DECIM, false, A>(a, t, ln, l, maxsteps);
+    if (FROM_CENSUS && t.dx == 0) {
+        // line i of a column direction is column x = i
+        if (base >= a.D - 1)
+            path_generic<DPL, FROM_CENSUS, SAT, DECIM, 2, A>(a, t, ln, l, maxsteps);
+        else
+            path_generic<DPL, FROM_CENSUS, SAT, DECIM, 1, A>(a, t, ln, l, maxsteps);
+    } else {
+        path_generic<DPL, FROM_CENSUS, SAT, DECIM, 0, A>(a, t, ln, l, maxsteps);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -291,11 +291,24 @@ static int plain_paths(sgmh_ctx *ctx, hipStream_t s, const uint64_t *cl, const u
     return launch_paths(s, a, dirs, ndirs, true, false, vol, dir_stride, 0);
 }
 
+// Census code buffers: (N + kLead) codes.  The kLead slack before pixel 0 is
+// read by disparity windows that start left of x = 0 on row 0; it must hold
+// valid (62-bit) codes there, so a new allocation gets it zeroed.
+static int ensure_codes(DevBuf &b, size_t N)
+{
+    const bool fresh = b.cap < (N + kLead) * 8;
+    if (ensure(b, (N + kLead) * 8))
+        return -1;
+    if (fresh)
+        CK(hipMemset(b.p, 0, (size_t)kLead * 8));
+    return 0;
+}
+
 static int plain_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W,
                          int H, int D, int paths, uint8_t *disp)
 {
     const size_t N = (size_t)W * H;
-    if (ensure(ctx->cl, (N + kLead) * 8) || ensure(ctx->cr, (N + kLead) * 8) ||
+    if (ensure_codes(ctx->cl, N) || ensure_codes(ctx->cr, N) ||
         ensure(ctx->vol, N * D * paths))
         return -1;
     uint64_t *cl = (uint64_t *)ctx->cl.p, *cr = (uint64_t *)ctx->cr.p; // padded bases
@@ -329,7 +342,7 @@ static int window_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const 
 {
     const size_t N = (size_t)W * H;
     REQUIRE(N * D < (1ull << 31), "sgmh_window: %dx%d D=%d exceeds the 2 GiB per-volume limit", W, H, D);
-    if (ensure(ctx->cl, (N + kLead) * 8) || ensure(ctx->cr, (N + kLead) * 8) || ensure(ctx->vol, N * D * 4))
+    if (ensure_codes(ctx->cl, N) || ensure_codes(ctx->cr, N) || ensure(ctx->vol, N * D * 4))
         return -1;
     uint64_t *cl = (uint64_t *)ctx->cl.p, *cr = (uint64_t *)ctx->cr.p;
     uint8_t *vol = (uint8_t *)ctx->vol.p;
@@ -668,7 +681,7 @@ int sgmh_path_plain_dev(sgmh_ctx *c, const uint64_t *cl, const uint64_t *cr, int
     begin_call(c, s);
     // copy the caller's dense codes behind the lead pad the kernels expect
     const size_t N = (size_t)W * H;
-    if (ensure(c->cl, (N + kLead) * 8) || ensure(c->cr, (N + kLead) * 8))
+    if (ensure_codes(c->cl, N) || ensure_codes(c->cr, N))
         return -1;
     uint64_t *pl = (uint64_t *)c->cl.p, *pr = (uint64_t *)c->cr.p;
     CK(hipMemcpyAsync(pl + kLead, cl, N * 8, hipMemcpyDeviceToDevice, s));

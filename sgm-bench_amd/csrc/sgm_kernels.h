@@ -70,7 +70,10 @@ __device__ __forceinline__ void sgm_step(uint32_t (&L)[DPL], const uint32_t (&C)
     const uint32_t below = dpp_or<kRowShr1>(0xFFFFFFFFu, a[DPL - 1]); // Lp(l*DPL-1)+P1
     const uint32_t above = dpp_or<kRowShl1>(0xFFFFFFFFu, a[0]);       // Lp(l*DPL+DPL)+P1
     const uint32_t jump = mn + P2;
-    const uint32_t neg = 0u - mn;
+    // -mn through an opaque VALU op: the compiler would fold `C + m + (0 - mn)`
+    // back into (C - mn) + m, two adds per disparity instead of one v_add3.
+    uint32_t neg;
+    asm("v_sub_u32 %0, 0, %1" : "=v"(neg) : "v"(mn));
     uint32_t lo = 0xFFFFFFFFu;
 #pragma unroll
     for (int j = 0; j < DPL; j++) {
@@ -84,6 +87,14 @@ __device__ __forceinline__ void sgm_step(uint32_t (&L)[DPL], const uint32_t (&C)
         lo = umin(lo, v);
     }
     mn = row_min16(lo);
+}
+
+// Four byte values (each < 256) into one word: two v_lshl_or_b32 and one
+// v_perm_b32 taking the low half of each (the perm also keeps the compiler
+// from re-associating the ors into four ops).
+__device__ __forceinline__ uint32_t pack4(uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    return __builtin_amdgcn_perm((d << 8) | c, (b << 8) | a, 0x05040100u);
 }
 
 template <int DPL>
@@ -109,7 +120,7 @@ __device__ __forceinline__ void store_bytes(uint8_t *dst, const uint32_t (&v)[DP
         uint32_t w[DPL / 4];
 #pragma unroll
         for (int k = 0; k < DPL / 4; k++)
-            w[k] = v[4 * k] | (v[4 * k + 1] << 8) | (v[4 * k + 2] << 16) | (v[4 * k + 3] << 24);
+            w[k] = pack4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
         if constexpr (DPL == 4) {
             *reinterpret_cast<uint32_t *>(dst) = w[0];
         } else if constexpr (DPL == 8) {
