@@ -669,20 +669,31 @@ constexpr int kWtaBatch = 2; // pixel batches per wta_batch call
 // ---------------------------------------------------------------------------
 // Each 16x16 tile t has an active range of 16-disparity blocks
 // [blk[t] & 15, blk[t] >> 4]; the recurrence sees only the predecessor's
-// active disparities (the rest are absent, +inf), mn is the minimum over the
-// predecessor's active range, and m(d) saturates at 255 as in the reference
-// step.  With every block active it is the plain path exactly.
+// active disparities (the rest are absent), mn is the minimum over the
+// predecessor's active range, each candidate of m(d) saturates at 255 as in
+// the reference step, and the argmin runs over the pixel's own range.  With
+// every block active it is the plain path exactly.
 //
-// Layout: lane l of a 16-lane row holds d = 16*b + l in register b, so a
-// window of k blocks keeps every lane busy with k registers, and a window that
-// moves at a tile boundary needs no lane realignment.  d-1 / d+1 are a row
-// rotation by one lane, with lane 0 / lane 15 taking the neighbouring block's
-// value.  The four scanlines of a wave share one tile (groups of 4 rows or 4
-// columns never straddle a 16-pixel tile), so the active-block test is a
-// scalar branch.
-constexpr uint32_t kWinAbsent = 0x3FFu;
-constexpr int kRowRor1 = 0x121;  // lane i <- lane (i-1) mod 16
-constexpr int kRowRor15 = 0x12F; // lane i <- lane (i+1) mod 16
+// Absent disparities are held as 255.  A real L never exceeds 254 (m <= mn +
+// P2, so L <= C + P2), so 255 is never a real value and never the minimum
+// while any real value exists, and as a candidate Lp(d) = 255 caps m(d) at 255
+// -- the reference's saturation -- with no extra instruction: the plain
+// sgm_step is exact for windows as it stands.
+//
+// A line is cut into 16-step segments, one per tile it crosses; the 4 lines of
+// a wave share every tile (groups of 4 rows or 4 columns never straddle one),
+// so a segment's block range is wave-uniform.  Three segment modes:
+//   NARROW  at most two blocks [b0, b0+1]: lane l holds d = 16*b0 + 2l + j,
+//           j = 0, 1 (the plain layout at DPL = 2 over a 32-disparity window)
+//   FULL    every block: the plain layout at DPL = D/16
+//   MASKED  other ranges: the plain layout, absent disparities pinned to 255
+// The state moves between layouts through a per-wave LDS row (one byte per
+// line and disparity) only where a segment's mode or window differs from the
+// previous one.  Every mode issues the same memory instructions per step (one
+// left load, D/32 16-byte right loads, one store), so the prefetch ring's
+// counters stay exact across mode changes.
+constexpr uint32_t kWinAbsent = 255u;
+constexpr uint32_t kWinPenalty = 1000u; // cost of an absent disparity inside a step: L > 255
 
 struct WinArgs {
     const uint64_t *cl, *cr; // census codes, pixel i at index kLead + i
@@ -693,92 +704,247 @@ struct WinArgs {
     DirTask dir[4];
 };
 
+struct WinSeg {
+    int mode; // 0 NARROW, 1 FULL, 2 MASKED
+    int b0;   // NARROW: first block
+    uint32_t mask;
+};
+
+template <int NB>
+__device__ __forceinline__ WinSeg win_seg(uint32_t rng)
+{
+    const int b1 = min((int)(rng >> 4), NB - 1), b0 = min((int)(rng & 15u), b1); // clamped
+    WinSeg w;
+    w.mask = ((2u << b1) - 1u) & ~((1u << b0) - 1u);
+    w.b0 = b0;
+    w.mode = b1 - b0 <= 1 ? 0 : (w.mask == (1u << NB) - 1u ? 1 : 2);
+    return w;
+}
+
 template <int NB, bool HORIZ>
 __device__ __forceinline__ void path_window_group(const WinArgs &a, const DirTask &t, int grp)
 {
+    constexpr int PF = 4;
+    constexpr int D = 16 * NB;
+    constexpr int NC = NB / 2; // 16-byte right loads per step (FULL layout)
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    __shared__ __align__(16) uint8_t xs[kLinesPerWave * D]; // layout exchange (one wave per workgroup)
     const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
-    const int W = a.W, H = a.H, D = a.D, dx = t.dx, dy = t.dy;
+    const int W = a.W, H = a.H;
+    const int dirp = HORIZ ? t.dx : t.dy;
+    const int len = HORIZ ? W : H;
+    const int nt = (len + 15) >> 4;
     const int line = grp * kLinesPerWave + g;
     const uint32_t live = line < t.nlines ? 0u : kDrop;
     const int ln = line < t.nlines ? line : t.nlines - 1;
-    int x = HORIZ ? (dx > 0 ? 0 : W - 1) : ln;
-    int y = HORIZ ? ln : (dy > 0 ? 0 : H - 1);
-    const int steps = HORIZ ? W : H;
-    // wave-uniform tile coordinate along the fixed axis
     const int fixed_tile = (grp * kLinesPerWave) >> 4;
     const uint32_t nb8 = (uint32_t)(W * H + kLead) * 8u;
     const rsrc_t rl = make_rsrc(a.cl, nb8), rr = make_rsrc(a.cr, nb8);
     const rsrc_t ro = make_rsrc(t.L, a.out_bytes);
     const uint32_t P1 = (uint32_t)t.P1, P2 = kPlainP2, inval = kPlainInvalid;
-    const bool lane0 = l == 0, lane15 = l == 15;
+    // Steps are counted on a tile-aligned axis q in [0, 16*nt): position
+    // p = q (forward) or 16*nt-1-q (backward); real steps have 0 <= p < len.
+    const int q0 = dirp > 0 ? 0 : 16 * nt - len, qend = dirp > 0 ? len : 16 * nt;
+    auto pos = [&](int q) { return dirp > 0 ? q : 16 * nt - 1 - q; };
+    auto pixel = [&](int p) { return HORIZ ? ln * W + p : p * W + ln; };
+    auto seg_raw = [&](int k) -> uint32_t {
+        const int kk = k < nt ? k : nt - 1;
+        const int along = dirp > 0 ? kk : nt - 1 - kk;
+        const int tx = HORIZ ? along : fixed_tile, ty = HORIZ ? fixed_tile : along;
+        return __builtin_amdgcn_readfirstlane((uint32_t)a.blk[ty * a.tiles_x + tx]);
+    };
 
-    uint32_t L[NB], mn = 0;
+    // ring slot k: the left code and the right codes of step q (q % PF == k)
+    uint64_t rcl[PF];
+    v4u rcr[PF][NC];
+    auto fetch = [&](int q, int k, const WinSeg &sg) {
+        const int pix = pixel(pos(q)) + kLead;
+        rcl[k] = ld64(rl, (uint32_t)pix * 8u);
+        // NARROW: codes of d = 16*b0 + 2l + {1, 0} in chunk 0, the other chunks
+        // repeat it; FULL/MASKED: codes of d = NB*l + NB-1-2i, NB-2-2i in chunk i
+        const int lo = sg.mode == 0 ? pix - 16 * sg.b0 - 2 * l - 1 : pix - NB * l - (NB - 1);
+        const int st = sg.mode == 0 ? 0 : 2;
 #pragma unroll
-    for (int b = 0; b < NB; b++)
-        L[b] = 0; // a virtual predecessor with all blocks at 0: the first step gives L = C
-    for (int s = 0; s < steps; s++) {
-        const int tx = HORIZ ? (x >> 4) : fixed_tile, ty = HORIZ ? fixed_tile : (y >> 4);
-        const uint32_t rng = __builtin_amdgcn_readfirstlane((uint32_t)a.blk[ty * a.tiles_x + tx]);
-        const int b1 = min((int)(rng >> 4), NB - 1), b0 = min((int)(rng & 15u), b1); // clamped: a bad table cannot index past the pixel
-        const int pix = y * W + x;
-        const uint64_t cL = ld64(rl, (uint32_t)(pix + kLead) * 8u);
-        uint64_t cR[NB];
+        for (int i = 0; i < NC; i++)
+            rcr[k][i] = __builtin_amdgcn_raw_buffer_load_b128(rr, (uint32_t)(lo + st * i) * 8u, 0, 0);
+    };
+    auto code = [&](int k, int i, int half) -> uint64_t {
+        const v4u v = rcr[k][i];
+        return half ? ((uint64_t)v[3] << 32) | v[2] : ((uint64_t)v[1] << 32) | v[0];
+    };
+
+    uint32_t SN[2] = {0, 0}, SF[NB], mn = 0;
 #pragma unroll
-        for (int b = 0; b < NB; b++)
-            if (b >= b0 && b <= b1)
-                cR[b] = ld64(rr, (uint32_t)(pix + kLead - 16 * b - l) * 8u);
-        uint32_t aP[NB];
+    for (int j = 0; j < NB; j++)
+        SF[j] = 0;
+    // Lp+P1 just below / above a NARROW window, for the first step after a
+    // change of window: the predecessor's range may reach past the new window.
+    uint32_t edge_lo = 0xFFFFFFFFu, edge_hi = 0xFFFFFFFFu;
+    // Move the state from the layout of segment `o` to that of segment `c`.
+    // The row is written and read with different access widths, so every
+    // access goes through memcpy (byte semantics: no type-based aliasing).
+    auto put = [](uint8_t *dst, const uint32_t *v, int n) {
+        uint8_t b[NB];
+        for (int j = 0; j < n; j++)
+            b[j] = (uint8_t)v[j];
+        __builtin_memcpy(dst, b, n);
+    };
+    auto get = [](const uint8_t *src, uint32_t *v, int n) {
+        uint8_t b[NB];
+        __builtin_memcpy(b, src, n);
+        for (int j = 0; j < n; j++)
+            v[j] = b[j];
+    };
+    auto exchange = [&](const WinSeg &o, const WinSeg &c) {
+        uint8_t *row = xs + g * D;
+        if (o.mode == 0) {
+            uint32_t ff[NB];
 #pragma unroll
-        for (int b = 0; b < NB; b++)
-            aP[b] = L[b] + P1;
-        const uint32_t jump = mn + P2;
-        uint32_t nL[NB], lo = 0xFFFFFFFFu;
-#pragma unroll
-        for (int b = 0; b < NB; b++) {
-            if (b >= b0 && b <= b1) {
-                const uint32_t h = (uint32_t)__popcll(cL ^ cR[b]);
-                const uint32_t C = x - 16 * b - l >= 0 ? h : inval;
-                const uint32_t r1 = dpp<kRowRor1>(aP[b]);
-                const uint32_t r1m = b > 0 ? dpp<kRowRor1>(aP[b - 1]) : 0xFFFFFFFFu;
-                const uint32_t r15 = dpp<kRowRor15>(aP[b]);
-                const uint32_t r15p = b < NB - 1 ? dpp<kRowRor15>(aP[b + 1]) : 0xFFFFFFFFu;
-                const uint32_t below = lane0 ? r1m : r1;
-                const uint32_t above = lane15 ? r15p : r15;
-                const uint32_t m = umin(umin(umin(L[b], jump), umin(below, above)), 255u);
-                nL[b] = C + m - mn;
-                lo = umin(lo, nL[b]);
+            for (int j = 0; j < NB; j++)
+                ff[j] = kWinAbsent;
+            put(row + NB * l, ff, NB);
+            const int d = 16 * o.b0 + 2 * l;
+            if (d < D)
+                put(row + d, SN, 2);
+        } else {
+            put(row + NB * l, SF, NB);
+        }
+        if (c.mode == 0) {
+            const int d = 16 * c.b0 + 2 * l;
+            if (d < D) {
+                get(row + d, SN, 2);
             } else {
-                nL[b] = kWinAbsent;
+                SN[0] = kWinAbsent;
+                SN[1] = kWinAbsent;
+            }
+            const int dl = 16 * c.b0 - 1, dh = 16 * c.b0 + 32;
+            edge_lo = dl >= 0 ? (uint32_t)row[dl] + P1 : 0xFFFFFFFFu;
+            edge_hi = dh < D ? (uint32_t)row[dh] + P1 : 0xFFFFFFFFu;
+        } else {
+            get(row + NB * l, SF, NB);
+        }
+    };
+
+    WinSeg cur = win_seg<NB>(seg_raw(0)), nxt = win_seg<NB>(seg_raw(1)), prev = cur;
+    uint32_t raw_nn = seg_raw(2);
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        fetch(k, k, cur);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    for (int k = 0; k < nt; k++) {
+        if (k > 0 && (cur.mode == 0) != (prev.mode == 0))
+            exchange(prev, cur);
+        else if (k > 0 && cur.mode == 0 && cur.b0 != prev.b0)
+            exchange(prev, cur);
+        const int qs = 16 * k;
+        if (cur.mode == 0) {
+            // NARROW: the second block may be absent (range of one block)
+            const bool one = !((cur.mask >> (cur.b0 + 1)) & 1u);
+            const uint32_t pen = one && l >= 8 ? kWinPenalty : 0u;
+            const int dbase = 16 * cur.b0 + 2 * l;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int q = qs + i, rk = i % PF;
+                const int p = pos(q);
+                const int xq = HORIZ ? p : ln;
+                uint32_t C[2];
+                const uint64_t cL = rcl[rk];
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const uint32_t h = (uint32_t)__popcll(cL ^ code(rk, 0, 1 - j));
+                    const uint32_t c = xq - dbase >= j ? h : inval;
+                    C[j] = c > pen ? c : pen;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                fetch(q + PF, rk, i + PF < 16 ? cur : nxt);
+                sgm_step<2, false>(SN, C, mn, P1, P2, edge_lo, edge_hi);
+                edge_lo = 0xFFFFFFFFu;
+                edge_hi = 0xFFFFFFFFu;
+                if (one) {
+                    SN[0] = umin(SN[0], kWinAbsent);
+                    SN[1] = umin(SN[1], kWinAbsent);
+                }
+                const bool real = q >= q0 && q < qend;
+                const uint32_t off = real ? ((uint32_t)pixel(p) * (uint32_t)D + (uint32_t)dbase) | live : kDrop;
+                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(SN[0] | (SN[1] << 8)), ro,
+                                                      dbase < D ? off : kDrop, 0, 0);
+                if (q < q0) { // before the line's first pixel (backward lines): keep the start state
+                    SN[0] = 0;
+                    SN[1] = 0;
+                    mn = 0;
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        } else {
+            const bool masked = cur.mode == 2;
+            uint32_t pen[NB];
+#pragma unroll
+            for (int j = 0; j < NB; j++)
+                pen[j] = masked && !((cur.mask >> ((NB * l + j) >> 4)) & 1u) ? kWinPenalty : 0u;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int q = qs + i, rk = i % PF;
+                const int p = pos(q);
+                const int xq = HORIZ ? p : ln;
+                uint32_t C[NB];
+                const uint64_t cL = rcl[rk];
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const int c16 = (NB - 1 - j) >> 1, half = (NB - 1 - j) & 1; // chunk, low/high
+                    const uint32_t h = (uint32_t)__popcll(cL ^ code(rk, c16, half));
+                    const uint32_t c = xq - NB * l >= j ? h : inval;
+                    C[j] = c > pen[j] ? c : pen[j];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                fetch(q + PF, rk, i + PF < 16 ? cur : nxt);
+                sgm_step<NB, false>(SF, C, mn, P1, P2);
+                if (masked) {
+#pragma unroll
+                    for (int j = 0; j < NB; j++)
+                        SF[j] = umin(SF[j], kWinAbsent);
+                }
+                const bool real = q >= q0 && q < qend;
+                const uint32_t off = real ? ((uint32_t)pixel(p) * (uint32_t)D + (uint32_t)(NB * l)) | live : kDrop;
+                st_bytes<NB>(ro, off, SF);
+                if (q < q0) {
+#pragma unroll
+                    for (int j = 0; j < NB; j++)
+                        SF[j] = 0;
+                    mn = 0;
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
-        mn = row_min16(lo);
-        const uint32_t base = (uint32_t)pix * (uint32_t)D + (uint32_t)l;
-#pragma unroll
-        for (int b = 0; b < NB; b++) {
-            L[b] = nL[b];
-            if (b >= b0 && b <= b1)
-                __builtin_amdgcn_raw_buffer_store_b8((uint8_t)nL[b], ro, (base + 16u * b) | live, 0, 0);
-        }
-        x += dx;
-        y += dy;
+        prev = cur;
+        cur = nxt;
+        nxt = win_seg<NB>(raw_nn);
+        raw_nn = seg_raw(k + 3);
     }
 }
 
-// Sum over the directions and argmin over the pixel's active range; one
-// thread per pixel, 16 disparities per load.
+// Sum over the directions and argmin over the pixel's active range.  One lane
+// per 16-disparity block (NB lanes per pixel, 64/NB pixels per wave): a lane
+// whose block is inactive loads nothing and bids the largest key.
 template <int NB>
-__device__ __forceinline__ void wta_window_px(const uint8_t *__restrict__ L, size_t dir_stride,
-                                              int ndirs, const uint8_t *__restrict__ blk, int tiles_x,
-                                              int W, int p, uint8_t *__restrict__ disp)
+__device__ __forceinline__ void wta_window_batch(const uint8_t *__restrict__ L, size_t dir_stride, int ndirs,
+                                                 const uint8_t *__restrict__ blk, int tiles_x, int W, int npix,
+                                                 uint8_t *__restrict__ disp, int b0)
 {
-    const int y = p / W, x = p - y * W;
+    constexpr int D = 16 * NB;
+    const int lane = threadIdx.x & 63, sub = lane % NB, slot = lane / NB;
+    const int p = b0 + slot;
+    const int pc = p < npix ? p : npix - 1;
+    const int y = pc / W, x = pc - y * W;
     const uint32_t rng = blk[(y >> 4) * tiles_x + (x >> 4)];
-    const int b1 = min((int)(rng >> 4), NB - 1), b0 = min((int)(rng & 15u), b1);
-    uint32_t best = 0xFFFFFFFFu;
-    for (int b = b0; b <= b1; b++) {
+    const int b1 = min((int)(rng >> 4), NB - 1), lo = min((int)(rng & 15u), b1);
+    uint32_t key = 0xFFFFFFFFu;
+    if (sub >= lo && sub <= b1) {
         uint32_t ev[4] = {0, 0, 0, 0}, od[4] = {0, 0, 0, 0};
+        const uint8_t *src = L + (size_t)pc * D + 16 * sub;
         for (int k = 0; k < ndirs; k++) {
-            const uint4 q = *reinterpret_cast<const uint4 *>(L + k * dir_stride + (size_t)p * (16 * NB) + 16 * b);
+            const uint4 q = *reinterpret_cast<const uint4 *>(src + k * dir_stride);
             const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -786,15 +952,17 @@ __device__ __forceinline__ void wta_window_px(const uint8_t *__restrict__ L, siz
                 od[i] += (w[i] >> 8) & 0x00FF00FFu;
             }
         }
-        const uint32_t d0 = 16u * b;
+        const uint32_t d0 = 16u * sub;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const uint32_t db = d0 + 4 * i;
-            best = umin(best, umin(((ev[i] & 0xFFFFu) << 8) | db, ((od[i] & 0xFFFFu) << 8) | (db + 1)));
-            best = umin(best, umin(((ev[i] >> 16) << 8) | (db + 2), ((od[i] >> 16) << 8) | (db + 3)));
+            key = umin(key, umin(((ev[i] & 0xFFFFu) << 8) | db, ((od[i] & 0xFFFFu) << 8) | (db + 1)));
+            key = umin(key, umin(((ev[i] >> 16) << 8) | (db + 2), ((od[i] >> 16) << 8) | (db + 3)));
         }
     }
-    disp[p] = (uint8_t)(best & 0xFFu);
+    key = group_min<NB>(key);
+    if (sub == 0 && p < npix)
+        disp[p] = (uint8_t)(key & 0xFFu);
 }
 
 } // namespace sgmk

@@ -134,8 +134,11 @@ __global__ __launch_bounds__(256) void k_wta_window(const uint8_t *__restrict__ 
                                                     const uint8_t *__restrict__ blk, int tiles_x, int W, int npix,
                                                     uint8_t *__restrict__ disp)
 {
-    for (int p = blockIdx.x * 256 + threadIdx.x; p < npix; p += gridDim.x * 256)
-        wta_window_px<NB>(L, dir_stride, ndirs, blk, tiles_x, W, p, disp);
+    constexpr int PPW = 64 / NB;
+    const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int nwaves = (int)gridDim.x * 4;
+    for (int b0 = wave * PPW; b0 < npix; b0 += nwaves * PPW)
+        wta_window_batch<NB>(L, dir_stride, ndirs, blk, tiles_x, W, npix, disp, b0);
 }
 
 // ---------------------------------------------------------------------------
@@ -384,7 +387,7 @@ static int window_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const 
     }
     {
         PhaseScope ps(ctx, s, PH_ARGMIN);
-        const int wb = (int)std::min<size_t>((N + 255) / 256, 16384);
+        const int wb = (int)std::min<size_t>((N + (size_t)(64 / (D / 16)) * 4 - 1) / ((size_t)(64 / (D / 16)) * 4), 16384);
         switch (D / 16) {
         case 2: hipLaunchKernelGGL(k_wta_window<2>, dim3(wb), dim3(256), 0, s, vol, N * D, 4, blk, a.tiles_x, W, (int)N, disp); break;
         case 4: hipLaunchKernelGGL(k_wta_window<4>, dim3(wb), dim3(256), 0, s, vol, N * D, 4, blk, a.tiles_x, W, (int)N, disp); break;

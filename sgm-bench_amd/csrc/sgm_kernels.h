@@ -59,16 +59,20 @@ __device__ __forceinline__ uint32_t row_min16(uint32_t v)
 // result without the clamps.  Out-of-range neighbours (d = -1, d = D) are
 // absent, which the DPP `old` value 0xFFFFFFFF expresses.  SAT clamps the
 // final sum at 255 (multiscale rule; the plain rule cannot exceed 254).
+//
+// edge_lo / edge_hi: Lp+P1 of the disparities just below / above the row's
+// range, seen by lane 0 / lane 15 (0xFFFFFFFF: none).
 template <int DPL, bool SAT>
 __device__ __forceinline__ void sgm_step(uint32_t (&L)[DPL], const uint32_t (&C)[DPL],
-                                         uint32_t &mn, uint32_t P1, uint32_t P2)
+                                         uint32_t &mn, uint32_t P1, uint32_t P2,
+                                         uint32_t edge_lo = 0xFFFFFFFFu, uint32_t edge_hi = 0xFFFFFFFFu)
 {
     uint32_t a[DPL];
 #pragma unroll
     for (int j = 0; j < DPL; j++)
         a[j] = L[j] + P1;
-    const uint32_t below = dpp_or<kRowShr1>(0xFFFFFFFFu, a[DPL - 1]); // Lp(l*DPL-1)+P1
-    const uint32_t above = dpp_or<kRowShl1>(0xFFFFFFFFu, a[0]);       // Lp(l*DPL+DPL)+P1
+    const uint32_t below = dpp_or<kRowShr1>(edge_lo, a[DPL - 1]); // Lp(l*DPL-1)+P1
+    const uint32_t above = dpp_or<kRowShl1>(edge_hi, a[0]);       // Lp(l*DPL+DPL)+P1
     const uint32_t jump = mn + P2;
     // -mn through an opaque VALU op: the compiler would fold `C + m + (0 - mn)`
     // back into (C - mn) + m, two adds per disparity instead of one v_add3.
