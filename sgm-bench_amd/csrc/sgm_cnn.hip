@@ -22,6 +22,7 @@
 #include <string.h>
 #include <vector>
 
+#include "sgm_device.h"
 #include "sgm_internal.h"
 
 namespace {
@@ -60,24 +61,44 @@ __global__ __launch_bounds__(256) void k_blocksum(const uint8_t *__restrict__ im
     out[i] = s;
 }
 
-// f_k = sum_{clamped 3x3} |Ld - Rd(x'-k)| (4080 where x'-k < 0) / (9*4080), bf16
+// f_k = sum_{clamped 3x3} |Ld - Rd(x'-k)| (4080 where x'-k < 0) / (9*4080), bf16.
+// A workgroup makes 32 pixels of one row, all 32 bins: the three rows of Ld
+// it needs (x0-1 .. x0+32) and of Rd (x0-32 .. x0+32) are staged in LDS;
+// thread (k, s) makes bin k of pixels x0 + 4s .. x0 + 4s + 3, so the 32
+// threads of one pixel write its 64 bytes contiguously.
+constexpr int kFeatTX = 32;
 __global__ __launch_bounds__(256) void k_features(const int32_t *__restrict__ Ld, const int32_t *__restrict__ Rd,
                                                   int w, int h, uint16_t *__restrict__ X)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= w * h * kC)
-        return;
-    const int k = i % kC, p = i / kC, y = p / w, x = p - y * w;
-    int acc = 0;
-    for (int dy = -1; dy <= 1; dy++) {
-        const int yy = min(max(y + dy, 0), h - 1);
-        for (int dx = -1; dx <= 1; dx++) {
-            const int xx = min(max(x + dx, 0), w - 1);
-            const int a = Ld[yy * w + xx];
-            acc += xx - k >= 0 ? abs(a - Rd[yy * w + xx - k]) : 4080;
-        }
+    __shared__ int32_t sl[3][kFeatTX + 2], sr[3][kFeatTX + 33];
+    const int x0 = blockIdx.x * kFeatTX, y = blockIdx.y;
+    for (int i = threadIdx.x; i < 3 * (kFeatTX + 2); i += 256) {
+        const int r = i / (kFeatTX + 2), c = i - r * (kFeatTX + 2);
+        const int yy = min(max(y + r - 1, 0), h - 1), xx = min(max(x0 - 1 + c, 0), w - 1);
+        sl[r][c] = Ld[yy * w + xx];
     }
-    X[i] = to_bf16((float)acc / (float)(9 * 4080));
+    for (int i = threadIdx.x; i < 3 * (kFeatTX + 33); i += 256) {
+        const int r = i / (kFeatTX + 33), c = i - r * (kFeatTX + 33);
+        const int yy = min(max(y + r - 1, 0), h - 1), xx = min(max(x0 - 32 + c, 0), w - 1);
+        sr[r][c] = Rd[yy * w + xx];
+    }
+    __syncthreads();
+    const int k = threadIdx.x & 31, sub = threadIdx.x >> 5;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int x = x0 + 4 * sub + u;
+        if (x >= w)
+            break;
+        int acc = 0;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const int xx = min(max(x + dx, 0), w - 1);
+                acc += xx - k >= 0 ? abs(sl[r][xx - x0 + 1] - sr[r][xx - k - x0 + 32]) : 4080;
+            }
+        X[((size_t)y * w + x) * kC + k] = to_bf16((float)acc / (float)(9 * 4080));
+    }
 }
 
 // ---- convolution (implicit GEMM on MFMA) ----
@@ -85,31 +106,58 @@ __global__ __launch_bounds__(256) void k_features(const int32_t *__restrict__ Ld
 // mode 1: Y = conv + b as fp32            (logits)
 // mode 2: Y = conv * (mask > 0) as bf16    (backward data through a ReLU)
 // Wp: packed bf16 B operands, [co][k] with k = tap*32 + ci, 32 x 288.
-__global__ __launch_bounds__(64) void k_conv(const uint16_t *__restrict__ X, int w, int h,
-                                             const uint16_t *__restrict__ Wp, const float *__restrict__ bias,
-                                             int mode, const uint16_t *__restrict__ mask, void *__restrict__ Y)
+// A workgroup of 4 waves stages the layer's 18 KB of B operands in LDS once;
+// each wave keeps its copy in registers and walks 16-pixel row tiles, loading
+// the next tile's 18 A operands while the MFMAs of the current one run.
+// Out-of-image taps read through an out-of-range buffer offset (zeros), so
+// the tile loop has no divergent loads.
+constexpr int kConvWaves = 4;
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ X, int w, int h,
+                                              const uint16_t *__restrict__ Wp, const float *__restrict__ bias,
+                                              const uint16_t *__restrict__ mask, void *__restrict__ Y)
 {
-    const int l = threadIdx.x, lr = l & 15, lq = l >> 4;
+    __shared__ uint2 wlds[kC * 288 / 4]; // 32 x 288 bf16
+    for (int i = threadIdx.x; i < kC * 288 / 4; i += 256)
+        wlds[i] = reinterpret_cast<const uint2 *>(Wp)[i];
+    __syncthreads();
+    const int l = threadIdx.x & 63, lr = l & 15, lq = l >> 4, wv = threadIdx.x >> 6;
     s16x4 B[18][2];
 #pragma unroll
     for (int kb = 0; kb < 18; kb++)
 #pragma unroll
-        for (int nt = 0; nt < 2; nt++) {
-            const uint2 v = *reinterpret_cast<const uint2 *>(Wp + (size_t)(16 * nt + lr) * 288 + 16 * kb + 4 * lq);
-            B[kb][nt] = __builtin_bit_cast(s16x4, v);
-        }
+        for (int nt = 0; nt < 2; nt++)
+            B[kb][nt] = __builtin_bit_cast(s16x4, wlds[((16 * nt + lr) * 288 + 16 * kb + 4 * lq) / 4]);
     const int tiles_x = (w + 15) / 16, ntiles = tiles_x * h;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const sgmk::rsrc_t rx = sgmk::make_rsrc(X, (uint32_t)((size_t)w * h * kC * 2));
+    auto load_tile = [&](int t, uint2 (&A)[18]) {
         const int y = t / tiles_x, x0 = (t - y * tiles_x) * 16;
-        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kb = 0; kb < 18; kb++) {
             const int tap = kb >> 1, ky = tap / 3, kx = tap % 3, ci = (kb & 1) * 16 + 4 * lq;
             const int xx = x0 + lr + kx - 1, yy = y + ky - 1;
-            uint2 v = make_uint2(0u, 0u);
-            if (xx >= 0 && xx < w && yy >= 0 && yy < h)
-                v = *reinterpret_cast<const uint2 *>(X + ((size_t)yy * w + xx) * kC + ci);
-            const s16x4 a = __builtin_bit_cast(s16x4, v);
+            const bool in = t < ntiles && xx >= 0 && xx < w && yy >= 0 && yy < h;
+            const uint32_t off = (uint32_t)(((yy * w + xx) * kC + ci) * 2);
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(rx, in ? off : sgmk::kDrop, 0, 0);
+            A[kb] = make_uint2(v[0], v[1]);
+        }
+    };
+    float bch[2] = {0.f, 0.f};
+    if constexpr (MODE != 2) {
+        bch[0] = bias[lr];
+        bch[1] = bias[16 + lr];
+    }
+    const int stride = gridDim.x * kConvWaves;
+    int t = blockIdx.x * kConvWaves + wv;
+    uint2 A[18], An[18];
+    load_tile(t, A);
+    for (; t < ntiles; t += stride) {
+        load_tile(t + stride, An);
+        const int y = t / tiles_x, x0 = (t - y * tiles_x) * 16;
+        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 18; kb++) {
+            const s16x4 a = __builtin_bit_cast(s16x4, A[kb]);
             acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[kb][0], acc0, 0, 0, 0);
             acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[kb][1], acc1, 0, 0, 0);
         }
@@ -125,17 +173,20 @@ __global__ __launch_bounds__(64) void k_conv(const uint16_t *__restrict__ X, int
                     continue;
                 const size_t o = ((size_t)y * w + x) * kC + ch;
                 float v = acc[i];
-                if (mode == 0) {
-                    v = fmaxf(v + bias[ch], 0.f);
+                if constexpr (MODE == 0) {
+                    v = fmaxf(v + bch[nt], 0.f);
                     reinterpret_cast<uint16_t *>(Y)[o] = to_bf16(v);
-                } else if (mode == 1) {
-                    reinterpret_cast<float *>(Y)[o] = v + bias[ch];
+                } else if constexpr (MODE == 1) {
+                    reinterpret_cast<float *>(Y)[o] = v + bch[nt];
                 } else {
                     v = from_bf16(mask[o]) > 0.f ? v : 0.f;
                     reinterpret_cast<uint16_t *>(Y)[o] = to_bf16(v);
                 }
             }
         }
+#pragma unroll
+        for (int kb = 0; kb < 18; kb++)
+            A[kb] = An[kb];
     }
 }
 
@@ -409,7 +460,7 @@ int cnn_pack(sgmh_cnn *c, hipStream_t s)
     return 0;
 }
 
-int conv_blocks(int w, int h) { return std::min(((w + 15) / 16) * h, 2048); }
+int conv_blocks(int w, int h) { return std::max(1, std::min((((w + 15) / 16) * h + kConvWaves - 1) / kConvWaves, 512)); }
 
 // features + 6 layers; logits in c->logits
 int cnn_forward(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, hipStream_t s)
@@ -419,12 +470,17 @@ int cnn_forward(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, h
         return -1;
     hipLaunchKernelGGL(k_blocksum, dim3((np + 255) / 256), dim3(256), 0, s, l, W, H, c->Ld);
     hipLaunchKernelGGL(k_blocksum, dim3((np + 255) / 256), dim3(256), 0, s, r, W, H, c->Rd);
-    hipLaunchKernelGGL(k_features, dim3((np * kC + 255) / 256), dim3(256), 0, s, c->Ld, c->Rd, w, h, c->act[0]);
+    hipLaunchKernelGGL(k_features, dim3((w + kFeatTX - 1) / kFeatTX, h), dim3(256), 0, s, c->Ld, c->Rd, w, h, c->act[0]);
     for (int i = 0; i < kLayers; i++) {
         const bool last = i == kLayers - 1;
-        hipLaunchKernelGGL(k_conv, dim3(conv_blocks(w, h)), dim3(64), 0, s, c->act[i], w, h, c->Wf + i * kWPer,
-                           c->theta + i * kPPer + kWPer, last ? 1 : 0, (const uint16_t *)nullptr,
-                           last ? (void *)c->logits : (void *)c->act[i + 1]);
+        if (last)
+            hipLaunchKernelGGL(k_conv<1>, dim3(conv_blocks(w, h)), dim3(256), 0, s, c->act[i], w, h,
+                               c->Wf + i * kWPer, c->theta + i * kPPer + kWPer, (const uint16_t *)nullptr,
+                               (void *)c->logits);
+        else
+            hipLaunchKernelGGL(k_conv<0>, dim3(conv_blocks(w, h)), dim3(256), 0, s, c->act[i], w, h,
+                               c->Wf + i * kWPer, c->theta + i * kPPer + kWPer, (const uint16_t *)nullptr,
+                               (void *)c->act[i + 1]);
     }
     CK(hipGetLastError());
     return 0;
@@ -443,8 +499,8 @@ int cnn_backward(sgmh_cnn *c, int W, int H, hipStream_t s)
         hipLaunchKernelGGL(k_bias_grad, dim3(std::min((np + 7) / 8, 1024)), dim3(256), 0, s, dy, np, gb);
         if (i > 0) {
             // dX = conv(dY, flipped / transposed W) * (act[i] > 0)
-            hipLaunchKernelGGL(k_conv, dim3(conv_blocks(w, h)), dim3(64), 0, s, dy, w, h, c->Wb + i * kWPer,
-                               (const float *)nullptr, 2, c->act[i], (void *)dx);
+            hipLaunchKernelGGL(k_conv<2>, dim3(conv_blocks(w, h)), dim3(256), 0, s, dy, w, h, c->Wb + i * kWPer,
+                               (const float *)nullptr, c->act[i], (void *)dx);
             std::swap(dy, dx);
         }
     }
