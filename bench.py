@@ -11,6 +11,10 @@ A step is one frame of the named workload, inputs already resident in HBM:
   batch  24 pairs of uneven size in one submission (8x640x480 D64, 8x1280x720
      D64, 4x1920x800 D128 multiscale, 4x1920x1080 D128), one graph launch per
      step; frames/s against a loop of single-pair launches of the same pairs
+  learned  1920x1080 D=128 4 paths: window proposal by the trained network,
+     then SGM over each tile's 32-disparity window (full D below confidence
+     0.9); frames/s against plain D=128 on the same frames, and the share of
+     pixels whose disparity differs from plain
 The last frame of the timed loop is hash-checked against the compiled
 reference oracle's golden for that scene; a mismatch exits 2, as the
 reference harness does (common/harness.c:236).
@@ -40,7 +44,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="A", choices=["A", "P", "B", "C", "batch"])
+    ap.add_argument("--workload", default="A", choices=["A", "P", "B", "C", "batch", "learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="approximate CPU time budget per CPU-baseline leg")
@@ -207,16 +211,148 @@ def bench_batch(a):
     return 0 if rec["golden_match"] else 2
 
 
+def bench_learned(a):
+    import numpy as np
+    import torch
+    import sgm_amd
+    from sgm_amd import learned
+
+    W, H, D, thresh = 1920, 1080, 128, 0.9
+    nframes = 4
+    ctx = sgm_amd.Context(0)
+    theta = learned.load_params()
+    net = sgm_amd.Cnn(ctx, theta)
+    sp = torch.cuda.current_stream().cuda_stream
+    T = ((H + 15) // 16) * ((W + 15) // 16)
+    frames = []
+    for i in range(nframes):
+        L, R = learned.scene(learned.EVAL_SEED0 + i, W, H)
+        frames.append((L, R, torch.from_numpy(L).cuda(), torch.from_numpy(R).cuda()))
+    d_plain = torch.empty((H, W), dtype=torch.uint8, device="cuda")
+    d_win = torch.empty_like(d_plain)
+    d_blk = torch.empty(T, dtype=torch.uint8, device="cuda")
+
+    def learned_step(k):
+        _, _, dl, dr = frames[k % nframes]
+        net.propose_dev(dl.data_ptr(), dr.data_ptr(), W, H, thresh, d_blk.data_ptr(), 0, sp)
+        ctx.window_dev(dl.data_ptr(), dr.data_ptr(), W, H, D, d_blk.data_ptr(), d_win.data_ptr(), sp)
+
+    def plain_step(k):
+        _, _, dl, dr = frames[k % nframes]
+        ctx.plain_dev(dl.data_ptr(), dr.data_ptr(), W, H, D, 4, d_plain.data_ptr(), sp)
+
+    def timed(fn):
+        for k in range(a.warmup):
+            fn(k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(a.steps):
+            fn(k)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    t_learned = timed(learned_step)
+    t_plain = timed(plain_step)
+    # quality and the per-frame algorithmic bytes of the windowed path kernel
+    diffs, mads, falls, act_bytes = [], [], [], []
+    for k in range(nframes):
+        plain_step(k)
+        learned_step(k)
+        torch.cuda.synchronize()
+        p, w, b = d_plain.cpu().numpy(), d_win.cpu().numpy(), d_blk.cpu().numpy()
+        diffs.append(float((p != w).mean() * 100))
+        mads.append(float(np.abs(p.astype(int) - w.astype(int)).mean()))
+        falls.append(float(((b >> 4) - (b & 15) == 7).mean() * 100))
+        width = ((b >> 4).astype(np.int64) - (b & 15) + 1).reshape((H + 15) // 16, (W + 15) // 16)
+        per_px = np.repeat(np.repeat(width, 16, 0), 16, 1)[:H, :W] * 16
+        act_bytes.append(4 * int(per_px.sum()) + 4 * 16 * W * H)
+    # phase split, HIP events on the launch stream
+    ctx.set_timing(True)
+    acc = {"census_ms": 0.0, "aggregate_ms": 0.0, "argmin_ms": 0.0}
+    nprof = 20
+    for k in range(nprof):
+        _, _, dl, dr = frames[k % nframes]
+        net.propose_dev(dl.data_ptr(), dr.data_ptr(), W, H, thresh, d_blk.data_ptr(), 0, sp)
+        ctx.window_dev(dl.data_ptr(), dr.data_ptr(), W, H, D, d_blk.data_ptr(), d_win.data_ptr(), sp)
+        t = ctx.times()
+        for key in acc:
+            acc[key] += max(t[key], 0.0)
+    ctx.set_timing(False)
+    phase = {k: v / nprof for k, v in acc.items()}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    _, _, dl, dr = frames[0]
+    e0.record()
+    for _ in range(nprof):
+        net.propose_dev(dl.data_ptr(), dr.data_ptr(), W, H, thresh, d_blk.data_ptr(), 0, sp)
+    e1.record()
+    torch.cuda.synchronize()
+    phase["proposal_ms"] = e0.elapsed_time(e1) / nprof
+    kb = int(np.mean(act_bytes))
+    agg_s = phase["aggregate_ms"] / 1000.0
+    fps = a.steps / t_learned
+    fps_plain = a.steps / t_plain
+    rec = {
+        "metric": "learned-window frames/s, 1080p D=128 4-path, vs plain D=128",
+        "value": round(fps, 1),
+        "unit": "frames/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(1000 * t_learned / a.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 network, u8/u16 integer SGM",
+        "data": "synthetic (held-out slanted-planes scenes, seeds %d..%d)" % (learned.EVAL_SEED0,
+                                                                           learned.EVAL_SEED0 + nframes - 1),
+        "config": {"workload": "learned: proposal network + windowed SGM, 1920x1080 D=128 4-path, "
+                               "confidence threshold 0.9", "W": W, "H": H, "D": D, "thresh": thresh},
+        "plain_d128_frames_per_s": round(fps_plain, 1),
+        "speedup_vs_plain": round(fps / fps_plain, 3),
+        "pct_pixels_differ": round(float(np.mean(diffs)), 3),
+        "mean_abs_diff": round(float(np.mean(mads)), 4),
+        "pct_tiles_full_D": round(float(np.mean(falls)), 2),
+        "phase_ms": {k: round(v, 4) for k, v in phase.items()},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_path_window (all directions, one launch)",
+            "achieved": round(kb / agg_s / 1e9, 1) if agg_s > 0 else None,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(kb / agg_s / 1e9 / HBM_PEAK_GBS, 4) if agg_s > 0 else None,
+            "traffic": None,
+            "algorithmic_bytes": kb,
+        },
+    }
+    if not a.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        import cnn_oracle
+        L, R = frames[0][0], frames[0][1]
+        t0 = time.perf_counter()
+        blk, _ = cnn_oracle.propose(theta, L, R, thresh)
+        oracle.sgm_window(L, R, blk, D, 4)
+        el = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(1.0 / el, 4), "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": "one 1080p frame: the network's numpy oracle (proposal) and the "
+                                         "scalar windowed-SGM oracle, %.1f s" % el}
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+    net.close()
+    ctx.close()
+    return 0
+
+
 def main():
     a = parse()
+    if a.workload == "learned":
+        if a.gpus != 1:
+            print("bench.py: one GPU only", file=sys.stderr)
+            return 1
+        return bench_learned(a)
     if a.workload == "batch":
         if a.gpus != 1:
             print("bench.py: one GPU only", file=sys.stderr)
             return 1
         return bench_batch(a)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not supported "
-              "(batched pairs would shard across GPUs by pair, see DESIGN.md).", file=sys.stderr)
+        print("bench.py: this engine runs on one GPU; multi-GPU runs are not supported.", file=sys.stderr)
         return 1
     import numpy as np
     import torch

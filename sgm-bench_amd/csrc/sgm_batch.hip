@@ -249,7 +249,8 @@ struct Schedule {
 
 } // namespace
 
-// The pairs of one class (plain or multiscale) and their phase work lists.
+// One lane: a subset of the pairs and their phase work lists, run as one
+// chain of phase kernels on its own stream.
 struct BatchLane {
     DevArray<PreTask> pre[3];
     DevArray<CostTask> cost[2];
@@ -262,20 +263,32 @@ struct BatchLane {
     bool empty() const { return path.host.empty(); }
 };
 
+// A partition of the pairs into lanes.  Lanes share no data, so their chains
+// run concurrently and one lane's memory-bound phases (census, cost volumes,
+// argmin) overlap another's arithmetic-bound path aggregation.
+struct LaneSet {
+    std::string name;
+    std::vector<BatchLane> lanes;
+};
+
+constexpr int kMaxLanes = 4;
+
 struct sgmh_batch {
     sgmh_ctx *ctx = nullptr;
     std::vector<sgmh_pair> pairs;
     void *arena = nullptr;
     size_t arena_bytes = 0;
-    BatchLane lane[2]; // 0: plain pairs, 1: multiscale pairs
+    std::vector<LaneSet> sets; // candidate partitions; sets[set] runs
+    int set = 0;
     int cus = 256;
     Schedule sched;
     std::string tuning; // JSON fragment
     bool graph_ok = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipStream_t cap = nullptr, cap2 = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t cap = nullptr;
+    hipStream_t side[kMaxLanes - 1] = {};
+    hipEvent_t fork = nullptr, join[kMaxLanes - 1] = {};
 };
 
 namespace {
@@ -353,19 +366,25 @@ int enqueue_lane(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc
     return 0;
 }
 
-// Both lanes, the multiscale one on a forked stream: they share no data, so
-// one lane's memory-bound phases overlap the other's path aggregation.
-int enqueue_phases(sgmh_batch *b, hipStream_t s, const Schedule &sc)
+// Every lane of the chosen set: lane 0 on `s`, the others on side streams
+// forked from it and joined back.
+int enqueue_phases(sgmh_batch *b, hipStream_t s, int set, const Schedule &sc)
 {
-    const bool two = !b->lane[0].empty() && !b->lane[1].empty();
-    if (!two)
-        return enqueue_lane(b, b->lane[b->lane[0].empty() ? 1 : 0], s, sc);
+    std::vector<BatchLane> &lanes = b->sets[set].lanes;
+    const int K = (int)lanes.size();
+    if (K == 1)
+        return enqueue_lane(b, lanes[0], s, sc);
     CK(hipEventRecord(b->fork, s));
-    CK(hipStreamWaitEvent(b->cap2, b->fork, 0));
-    if (enqueue_lane(b, b->lane[1], b->cap2, sc) || enqueue_lane(b, b->lane[0], s, sc))
+    for (int k = 1; k < K; k++) {
+        CK(hipStreamWaitEvent(b->side[k - 1], b->fork, 0));
+        if (enqueue_lane(b, lanes[k], b->side[k - 1], sc))
+            return -1;
+        CK(hipEventRecord(b->join[k - 1], b->side[k - 1]));
+    }
+    if (enqueue_lane(b, lanes[0], s, sc))
         return -1;
-    CK(hipEventRecord(b->join, b->cap2));
-    CK(hipStreamWaitEvent(s, b->join, 0));
+    for (int k = 1; k < K; k++)
+        CK(hipStreamWaitEvent(s, b->join[k - 1], 0));
     return 0;
 }
 
@@ -380,7 +399,7 @@ int capture(sgmh_batch *b)
         b->graph = nullptr;
     }
     CK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_phases(b, b->cap, b->sched);
+    const int rc = enqueue_phases(b, b->cap, b->set, b->sched);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->cap, &g);
     if (rc)
@@ -396,14 +415,20 @@ int capture(sgmh_batch *b)
 // the fastest.
 int autotune(sgmh_batch *b)
 {
-    std::vector<Schedule> cands;
-    for (int order = 0; order < 2; order++)
-        for (int pw = 1; pw <= 4; pw++) {
-            Schedule sc;
-            sc.order = order;
-            sc.path_waves_per_simd = pw;
-            cands.push_back(sc);
-        }
+    struct Cand {
+        int set;
+        Schedule sc;
+    };
+    std::vector<Cand> cands;
+    for (int set = 0; set < (int)b->sets.size(); set++)
+        for (int order = 0; order < 2; order++)
+            for (int pw : {1, 2, 4}) {
+                Cand c;
+                c.set = set;
+                c.sc.order = order;
+                c.sc.path_waves_per_simd = pw;
+                cands.push_back(c);
+            }
     hipEvent_t e0, e1;
     CK(hipEventCreate(&e0));
     CK(hipEventCreate(&e1));
@@ -412,25 +437,26 @@ int autotune(sgmh_batch *b)
     std::string js = "[";
     for (size_t c = 0; c < cands.size(); c++) {
         float t[3];
-        if (enqueue_phases(b, s, cands[c]))
+        if (enqueue_phases(b, s, cands[c].set, cands[c].sc))
             return -1;
         for (int r = 0; r < 3; r++) {
             CK(hipEventRecord(e0, s));
-            if (enqueue_phases(b, s, cands[c]))
+            if (enqueue_phases(b, s, cands[c].set, cands[c].sc))
                 return -1;
             CK(hipEventRecord(e1, s));
             CK(hipEventSynchronize(e1));
             CK(hipEventElapsedTime(&t[r], e0, e1));
         }
         std::sort(t, t + 3);
-        char buf[160];
-        snprintf(buf, sizeof buf, "%s{\"order\":\"%s\",\"path_waves_per_simd\":%d,\"ms\":%.4f}",
-                 c ? "," : "", cands[c].order == 0 ? "longest-first" : "pair-major",
-                 cands[c].path_waves_per_simd, t[1]);
+        char buf[200];
+        snprintf(buf, sizeof buf, "%s{\"lanes\":\"%s\",\"order\":\"%s\",\"path_waves_per_simd\":%d,\"ms\":%.4f}",
+                 c ? "," : "", b->sets[cands[c].set].name.c_str(),
+                 cands[c].sc.order == 0 ? "longest-first" : "pair-major", cands[c].sc.path_waves_per_simd, t[1]);
         js += buf;
         if (t[1] < best) {
             best = t[1];
-            b->sched = cands[c];
+            b->set = cands[c].set;
+            b->sched = cands[c].sc;
         }
     }
     js += "]";
@@ -527,167 +553,218 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
     }
     uint8_t *A = (uint8_t *)b->arena;
 
-    // ---- phase work lists ----
-    int pre_items[2][3] = {{0, 0, 0}, {0, 0, 0}};
-    int li = 0; // lane of the pair being added
-    auto add_pre = [&](int level, int kind, const uint8_t *src, void *dst, int W, int H, int ostride) {
-        PreTask t;
-        t.src = src;
-        t.dst = dst;
-        t.W = W;
-        t.H = H;
-        t.kind = kind;
-        t.ostride = ostride;
-        int ty;
-        if (kind == PRE_CENSUS) {
-            t.tiles_x = (W + kCenTX - 1) / kCenTX;
-            ty = (H + kCenTY - 1) / kCenTY;
-        } else {
-            const int ow = kind == PRE_HALF ? W / 2 : W, oh = kind == PRE_HALF ? H / 2 : H;
-            t.tiles_x = (ow + 63) / 64;
-            ty = (oh + 3) / 4;
+    // ---- phase work lists, one set per candidate partition of the pairs ----
+    auto build = [&](const std::vector<int> &lane_of, int nl, const char *name) -> int {
+        LaneSet ls;
+        ls.name = name;
+        ls.lanes.resize(nl);
+        int pre_items[kMaxLanes][3] = {}, cost_items[kMaxLanes][2] = {}, wta_items[kMaxLanes] = {};
+        int li = 0;
+        auto add_pre = [&](int level, int kind, const uint8_t *src, void *dst, int W, int H, int ostride) {
+            PreTask t;
+            t.src = src;
+            t.dst = dst;
+            t.W = W;
+            t.H = H;
+            t.kind = kind;
+            t.ostride = ostride;
+            int ty;
+            if (kind == PRE_CENSUS) {
+                t.tiles_x = (W + kCenTX - 1) / kCenTX;
+                ty = (H + kCenTY - 1) / kCenTY;
+            } else {
+                const int ow = kind == PRE_HALF ? W / 2 : W, oh = kind == PRE_HALF ? H / 2 : H;
+                t.tiles_x = (ow + 63) / 64;
+                ty = (oh + 3) / 4;
+            }
+            t.first_item = pre_items[li][level];
+            pre_items[li][level] += t.tiles_x * ty;
+            ls.lanes[li].pre[level].host.push_back(t);
+        };
+        auto add_cost = [&](int level, const uint64_t *cl, const uint64_t *cr, const uint8_t *merge, uint8_t *C,
+                            int W, int H, int D) {
+            CostTask t{cl, cr, merge, C, W, H, D, kMsInvalid, cost_items[li][level]};
+            cost_items[li][level] += (int)(((size_t)W * H + 15) / 16);
+            ls.lanes[li].cost[level].host.push_back(t);
+        };
+        auto add_wta = [&](const uint8_t *L, size_t stride, int ndirs, int npix, int D, uint8_t *disp) {
+            WtaTask t{L, stride, ndirs, npix, D, disp, wta_items[li]};
+            const int per = (64 / (D / 16)) * kWtaBatch;
+            wta_items[li] += (npix + per - 1) / per;
+            ls.lanes[li].wta.host.push_back(t);
+        };
+        auto add_path = [&](const uint64_t *cl, const uint64_t *cr, const uint8_t *cost, int W, int H, int D,
+                            uint32_t P2, uint32_t inval, uint32_t out_bytes, int dx, int dy, int P1, uint8_t *L,
+                            int variant) {
+            BatchPathTask t;
+            t.cl = cl;
+            t.cr = cr;
+            t.cost = cost;
+            t.W = W;
+            t.H = H;
+            t.D = D;
+            t.P2 = P2;
+            t.invalid = inval;
+            t.out_bytes = out_bytes;
+            t.dir.dx = dx;
+            t.dir.dy = dy;
+            t.dir.P1 = P1;
+            t.dir.nlines = dy == 0 ? H : (dx == 0 ? W : W + H - 1);
+            t.dir.first_block = 0;
+            t.dir.L = L;
+            t.variant = variant;
+            ls.lanes[li].path.host.push_back(t);
+        };
+        for (int i = 0; i < n; i++) {
+            const sgmh_pair &p = pairs[i];
+            const Ws &w = ws[i];
+            const int W = p.W, H = p.H, D = p.D, hw = W / 2, hh = H / 2;
+            const size_t N = (size_t)W * H, Nh = (size_t)hw * hh;
+            li = lane_of[i];
+            if (p.mode == SGMH_PLAIN) {
+                uint64_t *cl = (uint64_t *)(A + w.cl), *cr = (uint64_t *)(A + w.cr);
+                add_pre(0, PRE_CENSUS, p.d_left, cl + kLead, W, H, W);
+                add_pre(0, PRE_CENSUS, p.d_right, cr + kLead, W, H, W);
+                for (int k = 0; k < p.paths; k++)
+                    add_path(cl, cr, nullptr, W, H, D, kPlainP2, kPlainInvalid, (uint32_t)(N * D),
+                             kDirsPlain[k][0], kDirsPlain[k][1], (int)kPlainP1, A + w.vol + (size_t)k * N * D,
+                             plain_variant(D));
+                add_wta(A + w.vol, N * D, p.paths, (int)N, D, p.d_disp);
+            } else {
+                uint8_t *hl = A + w.hl, *hr = A + w.hr, *sl = A + w.sl, *sr = A + w.sr;
+                uint8_t *shl = A + w.shl, *shr = A + w.shr;
+                uint64_t *cl1 = (uint64_t *)(A + w.cl1), *cr1 = (uint64_t *)(A + w.cr1);
+                uint64_t *cl2 = (uint64_t *)(A + w.cl2), *cr2 = (uint64_t *)(A + w.cr2);
+                uint8_t *C1 = A + w.c1, *C2 = A + w.c2, *V1 = A + w.v1, *V2 = A + w.v2;
+                add_pre(0, PRE_HALF, p.d_left, hl, W, H, 0);
+                add_pre(0, PRE_HALF, p.d_right, hr, W, H, 0);
+                add_pre(0, PRE_SOBEL, p.d_left, sl, W, H, 0);
+                add_pre(0, PRE_SOBEL, p.d_right, sr, W, H, 0);
+                add_pre(1, PRE_SOBEL, hl, shl, hw, hh, 0);
+                add_pre(1, PRE_SOBEL, hr, shr, hw, hh, 0);
+                add_pre(1, PRE_CENSUS, sl, cl2, W, H, W);
+                add_pre(1, PRE_CENSUS, sr, cr2, W, H, W);
+                add_pre(2, PRE_CENSUS, shl, cl1, hw, hh, hw);
+                add_pre(2, PRE_CENSUS, shr, cr1, hw, hh, hw);
+                add_cost(0, cl1, cr1, nullptr, C1, hw, hh, D);
+                add_cost(1, cl2, cr2, C1, C2, W, H, D);
+                for (int k = 0; k < 8; k++) {
+                    add_path(nullptr, nullptr, C1, hw, hh, D, kMsP2, kMsInvalid, (uint32_t)(Nh * D),
+                             kDirsMs[k][0], kDirsMs[k][1], kDirsMs[k][2], V1 + (size_t)k * Nh * D,
+                             ms_variant(D, false));
+                    add_path(nullptr, nullptr, C2, W, H, D, kMsP2, kMsInvalid, (uint32_t)(Nh * D),
+                             kDirsMs[k][0], kDirsMs[k][1], kDirsMs[k][2], V2 + (size_t)k * Nh * D,
+                             ms_variant(D, true));
+                }
+                add_wta(V1, Nh * D, 8, (int)Nh, D, A + w.d1);
+                add_wta(V2, Nh * D, 8, (int)Nh, D, p.d_disp);
+            }
         }
-        t.first_item = pre_items[li][level];
-        pre_items[li][level] += t.tiles_x * ty;
-        b->lane[li].pre[level].host.push_back(t);
+        // drop empty lanes
+        std::vector<BatchLane> kept;
+        int kept_idx[kMaxLanes];
+        for (int L = 0; L < nl; L++)
+            if (!ls.lanes[L].path.host.empty()) {
+                kept_idx[kept.size()] = L;
+                kept.push_back(ls.lanes[L]);
+            }
+        for (size_t q = 0; q < kept.size(); q++) {
+            BatchLane &ln = kept[q];
+            const int L = kept_idx[q];
+            for (int k = 0; k < 3; k++)
+                ln.pre_n[k] = pre_items[L][k];
+            for (int k = 0; k < 2; k++)
+                ln.cost_n[k] = cost_items[L][k];
+            ln.wta_n = wta_items[L];
+            // path items in two orders: longest lines first, and pair by pair
+            std::vector<std::pair<long, uint32_t>> all;
+            for (size_t t = 0; t < ln.path.host.size(); t++) {
+                const BatchPathTask &tk = ln.path.host[t];
+                const int groups = (tk.dir.nlines + kLinesPerWave - 1) / kLinesPerWave;
+                for (int g = 0; g < groups; g++) {
+                    const uint32_t code = ((uint32_t)t << 16) | (uint32_t)g;
+                    ln.items_pair.push_back(code);
+                    all.push_back({group_work(tk, g), code});
+                }
+            }
+            REQUIRE(ln.path.host.size() < 65536, "sgmh_batch_plan: too many directions");
+            std::stable_sort(all.begin(), all.end(),
+                             [](const auto &x, const auto &y) { return x.first > y.first; });
+            for (auto &x : all)
+                ln.items_lpt.push_back(x.second);
+        }
+        ls.lanes = kept;
+        b->sets.push_back(ls);
+        return 0;
     };
-    int cost_items[2][2] = {{0, 0}, {0, 0}};
-    auto add_cost = [&](int level, const uint64_t *cl, const uint64_t *cr, const uint8_t *merge, uint8_t *C,
-                        int W, int H, int D) {
-        CostTask t{cl, cr, merge, C, W, H, D, kMsInvalid, cost_items[li][level]};
-        cost_items[li][level] += (int)(((size_t)W * H + 15) / 16);
-        b->lane[li].cost[level].host.push_back(t);
-    };
-    int wta_items[2] = {0, 0};
-    auto add_wta = [&](const uint8_t *L, size_t stride, int ndirs, int npix, int D, uint8_t *disp) {
-        WtaTask t{L, stride, ndirs, npix, D, disp, wta_items[li]};
-        const int per = (64 / (D / 16)) * kWtaBatch;
-        wta_items[li] += (npix + per - 1) / per;
-        b->lane[li].wta.host.push_back(t);
-    };
-    auto add_path = [&](const uint64_t *cl, const uint64_t *cr, const uint8_t *cost, int W, int H, int D,
-                        uint32_t P2, uint32_t inval, uint32_t out_bytes, int dx, int dy, int P1,
-                        uint8_t *L, int variant) {
-        BatchPathTask t;
-        t.cl = cl;
-        t.cr = cr;
-        t.cost = cost;
-        t.W = W;
-        t.H = H;
-        t.D = D;
-        t.P2 = P2;
-        t.invalid = inval;
-        t.out_bytes = out_bytes;
-        t.dir.dx = dx;
-        t.dir.dy = dy;
-        t.dir.P1 = P1;
-        t.dir.nlines = dy == 0 ? H : (dx == 0 ? W : W + H - 1);
-        t.dir.first_block = 0;
-        t.dir.L = L;
-        t.variant = variant;
-        b->lane[li].path.host.push_back(t);
-    };
+    // Candidate partitions: by configuration (plain / multiscale), and
+    // 2, 3, 4 lanes balanced by estimated work (longest pair first to the
+    // least-loaded lane).
+    std::vector<double> work(n);
     for (int i = 0; i < n; i++) {
         const sgmh_pair &p = pairs[i];
-        const Ws &w = ws[i];
-        const int W = p.W, H = p.H, D = p.D, hw = W / 2, hh = H / 2;
-        const size_t N = (size_t)W * H, Nh = (size_t)hw * hh;
-        li = p.mode == SGMH_PLAIN ? 0 : 1;
-        if (p.mode == SGMH_PLAIN) {
-            uint64_t *cl = (uint64_t *)(A + w.cl), *cr = (uint64_t *)(A + w.cr);
-            add_pre(0, PRE_CENSUS, p.d_left, cl + kLead, W, H, W);
-            add_pre(0, PRE_CENSUS, p.d_right, cr + kLead, W, H, W);
-            for (int k = 0; k < p.paths; k++)
-                add_path(cl, cr, nullptr, W, H, D, kPlainP2, kPlainInvalid, (uint32_t)(N * D),
-                         kDirsPlain[k][0], kDirsPlain[k][1], (int)kPlainP1, A + w.vol + (size_t)k * N * D,
-                         plain_variant(D));
-            add_wta(A + w.vol, N * D, p.paths, (int)N, D, p.d_disp);
-        } else {
-            uint8_t *hl = A + w.hl, *hr = A + w.hr, *sl = A + w.sl, *sr = A + w.sr;
-            uint8_t *shl = A + w.shl, *shr = A + w.shr;
-            uint64_t *cl1 = (uint64_t *)(A + w.cl1), *cr1 = (uint64_t *)(A + w.cr1);
-            uint64_t *cl2 = (uint64_t *)(A + w.cl2), *cr2 = (uint64_t *)(A + w.cr2);
-            uint8_t *C1 = A + w.c1, *C2 = A + w.c2, *V1 = A + w.v1, *V2 = A + w.v2;
-            add_pre(0, PRE_HALF, p.d_left, hl, W, H, 0);
-            add_pre(0, PRE_HALF, p.d_right, hr, W, H, 0);
-            add_pre(0, PRE_SOBEL, p.d_left, sl, W, H, 0);
-            add_pre(0, PRE_SOBEL, p.d_right, sr, W, H, 0);
-            add_pre(1, PRE_SOBEL, hl, shl, hw, hh, 0);
-            add_pre(1, PRE_SOBEL, hr, shr, hw, hh, 0);
-            add_pre(1, PRE_CENSUS, sl, cl2, W, H, W);
-            add_pre(1, PRE_CENSUS, sr, cr2, W, H, W);
-            add_pre(2, PRE_CENSUS, shl, cl1, hw, hh, hw);
-            add_pre(2, PRE_CENSUS, shr, cr1, hw, hh, hw);
-            add_cost(0, cl1, cr1, nullptr, C1, hw, hh, D);
-            add_cost(1, cl2, cr2, C1, C2, W, H, D);
-            for (int k = 0; k < 8; k++) {
-                add_path(nullptr, nullptr, C1, hw, hh, D, kMsP2, kMsInvalid, (uint32_t)(Nh * D),
-                         kDirsMs[k][0], kDirsMs[k][1], kDirsMs[k][2], V1 + (size_t)k * Nh * D,
-                         ms_variant(D, false));
-                add_path(nullptr, nullptr, C2, W, H, D, kMsP2, kMsInvalid, (uint32_t)(Nh * D),
-                         kDirsMs[k][0], kDirsMs[k][1], kDirsMs[k][2], V2 + (size_t)k * Nh * D,
-                         ms_variant(D, true));
-            }
-            add_wta(V1, Nh * D, 8, (int)Nh, D, A + w.d1);
-            add_wta(V2, Nh * D, 8, (int)Nh, D, p.d_disp);
-        }
+        const double nd = (double)p.W * p.H * p.D;
+        work[i] = p.mode == SGMH_PLAIN ? nd * p.paths * 1.5 : nd * 8 * 1.25 * 2.0;
     }
+    std::vector<int> by_work(n);
+    for (int i = 0; i < n; i++)
+        by_work[i] = i;
+    std::stable_sort(by_work.begin(), by_work.end(), [&](int x, int y) { return work[x] > work[y]; });
     int rc = 0;
-    for (int L = 0; L < 2 && !rc; L++) {
-        BatchLane &ln = b->lane[L];
-        for (int k = 0; k < 3; k++)
-            ln.pre_n[k] = pre_items[L][k];
-        for (int k = 0; k < 2; k++)
-            ln.cost_n[k] = cost_items[L][k];
-        ln.wta_n = wta_items[L];
-        // path items in two orders: longest lines first, and pair by pair
-        std::vector<std::pair<long, uint32_t>> all;
-        for (size_t t = 0; t < ln.path.host.size(); t++) {
-            const BatchPathTask &tk = ln.path.host[t];
-            const int groups = (tk.dir.nlines + kLinesPerWave - 1) / kLinesPerWave;
-            for (int g = 0; g < groups; g++) {
-                const uint32_t code = ((uint32_t)t << 16) | (uint32_t)g;
-                ln.items_pair.push_back(code);
-                all.push_back({group_work(tk, g), code});
+    {
+        std::vector<int> lane_of(n);
+        for (int i = 0; i < n; i++)
+            lane_of[i] = pairs[i].mode == SGMH_PLAIN ? 0 : 1;
+        rc = build(lane_of, 2, "by-configuration");
+    }
+    const bool tune = (flags & SGMH_BATCH_AUTOTUNE) != 0;
+    for (int K = 2; K <= kMaxLanes && tune && !rc && K <= n; K++) {
+        std::vector<int> lane_of(n);
+        std::vector<double> load(K, 0.0);
+        for (int i : by_work) {
+            const int L = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            lane_of[i] = L;
+            load[L] += work[i];
+        }
+        char name[32];
+        snprintf(name, sizeof name, "balanced-%d", K);
+        rc = build(lane_of, K, name);
+    }
+    for (LaneSet &ls : b->sets)
+        for (BatchLane &ln : ls.lanes) {
+            for (int k = 0; k < 3 && !rc; k++)
+                rc = upload(ln.pre[k]);
+            for (int k = 0; k < 2 && !rc; k++)
+                rc = upload(ln.cost[k]);
+            if (!rc)
+                rc = upload(ln.path);
+            if (!rc)
+                rc = upload(ln.wta);
+            if (!rc && !ln.items_lpt.empty()) {
+                const size_t m = ln.items_lpt.size();
+                if (hipMalloc(&ln.d_items, 2 * m * sizeof(uint32_t)) != hipSuccess ||
+                    hipMemcpy(ln.d_items, ln.items_lpt.data(), m * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(ln.d_items + m, ln.items_pair.data(), m * 4, hipMemcpyHostToDevice) != hipSuccess) {
+                    set_err("sgmh_batch_plan: cannot upload the path work list");
+                    rc = -1;
+                }
             }
-        }
-        if (ln.path.host.size() >= 65536) {
-            set_err("sgmh_batch_plan: too many directions");
-            rc = -1;
-            break;
-        }
-        std::stable_sort(all.begin(), all.end(), [](const auto &a, const auto &c) { return a.first > c.first; });
-        for (auto &x : all)
-            ln.items_lpt.push_back(x.second);
-        for (int k = 0; k < 3 && !rc; k++)
-            rc = upload(ln.pre[k]);
-        for (int k = 0; k < 2 && !rc; k++)
-            rc = upload(ln.cost[k]);
-        if (!rc)
-            rc = upload(ln.path);
-        if (!rc)
-            rc = upload(ln.wta);
-        if (!rc && !ln.items_lpt.empty()) {
-            const size_t m = ln.items_lpt.size();
-            if (hipMalloc(&ln.d_items, 2 * m * sizeof(uint32_t)) != hipSuccess ||
-                hipMemcpy(ln.d_items, ln.items_lpt.data(), m * 4, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(ln.d_items + m, ln.items_pair.data(), m * 4, hipMemcpyHostToDevice) != hipSuccess) {
-                set_err("sgmh_batch_plan: cannot upload the path work list");
+            if (!rc && hipMalloc(&ln.d_counter, 64) != hipSuccess) {
+                set_err("sgmh_batch_plan: cannot allocate the work counter");
                 rc = -1;
             }
         }
-        if (!rc && hipMalloc(&ln.d_counter, 64) != hipSuccess) {
-            set_err("sgmh_batch_plan: cannot allocate the work counter");
-            rc = -1;
-        }
-    }
     if (!rc && (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
-                hipStreamCreateWithFlags(&b->cap2, hipStreamNonBlocking) != hipSuccess ||
-                hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
-                hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)) {
+                hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)) {
         set_err("sgmh_batch_plan: cannot create streams / events");
         rc = -1;
     }
+    for (int k = 0; k < kMaxLanes - 1 && !rc; k++)
+        if (hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) != hipSuccess) {
+            set_err("sgmh_batch_plan: cannot create streams / events");
+            rc = -1;
+        }
     if (!rc && (flags & SGMH_BATCH_AUTOTUNE))
         rc = autotune(b);
     if (!rc && !(flags & SGMH_BATCH_NO_GRAPH))
@@ -713,7 +790,7 @@ int sgmh_batch_run(sgmh_batch *b, void *stream)
         CK(hipGraphLaunch(b->exec, s));
         return 0;
     }
-    return enqueue_phases(b, s, b->sched);
+    return enqueue_phases(b, s, b->set, b->sched);
 }
 
 int sgmh_batch_run_loop(sgmh_batch *b, void *stream)
@@ -736,16 +813,21 @@ int sgmh_batch_describe(const sgmh_batch *b, char *buf, size_t len)
     int plain = 0, ms = 0;
     for (const sgmh_pair &p : b->pairs)
         (p.mode == SGMH_PLAIN ? plain : ms)++;
+    size_t items = 0, tasks = 0, wtas = 0;
+    const LaneSet &ls = b->sets[b->set];
+    for (const BatchLane &ln : ls.lanes) {
+        items += ln.items_lpt.size();
+        tasks += ln.path.host.size();
+        wtas += ln.wta.host.size();
+    }
     const int n = snprintf(
         buf, len,
         "{\"pairs\":%zu,\"plain\":%d,\"multiscale\":%d,\"workspace_bytes\":%zu,\"path_items\":%zu,"
-        "\"path_tasks\":%zu,\"wta_tasks\":%zu,\"graph\":%s,\"schedule\":{\"order\":\"%s\","
-        "\"path_waves_per_simd\":%d},\"tuning\":%s}",
-        b->pairs.size(), plain, ms, b->arena_bytes, b->lane[0].items_lpt.size() + b->lane[1].items_lpt.size(),
-        b->lane[0].path.host.size() + b->lane[1].path.host.size(),
-        b->lane[0].wta.host.size() + b->lane[1].wta.host.size(), b->graph_ok ? "true" : "false",
-        b->sched.order == 0 ? "longest-first" : "pair-major", b->sched.path_waves_per_simd,
-        b->tuning.empty() ? "null" : b->tuning.c_str());
+        "\"path_tasks\":%zu,\"wta_tasks\":%zu,\"graph\":%s,\"schedule\":{\"lanes\":\"%s\",\"lane_count\":%zu,"
+        "\"order\":\"%s\",\"path_waves_per_simd\":%d},\"tuning\":%s}",
+        b->pairs.size(), plain, ms, b->arena_bytes, items, tasks, wtas, b->graph_ok ? "true" : "false",
+        ls.name.c_str(), ls.lanes.size(), b->sched.order == 0 ? "longest-first" : "pair-major",
+        b->sched.path_waves_per_simd, b->tuning.empty() ? "null" : b->tuning.c_str());
     return n;
 }
 
@@ -759,7 +841,8 @@ void sgmh_batch_destroy(sgmh_batch *b)
         (void)hipGraphExecDestroy(b->exec);
     if (b->graph)
         (void)hipGraphDestroy(b->graph);
-    for (BatchLane &ln : b->lane) {
+    for (LaneSet &ls : b->sets)
+        for (BatchLane &ln : ls.lanes) {
         for (auto &a : ln.pre)
             if (a.dev)
                 (void)hipFree(a.dev);
@@ -774,17 +857,19 @@ void sgmh_batch_destroy(sgmh_batch *b)
             (void)hipFree(ln.d_items);
         if (ln.d_counter)
             (void)hipFree(ln.d_counter);
-    }
+        }
     if (b->arena)
         (void)hipFree(b->arena);
     if (b->cap)
         (void)hipStreamDestroy(b->cap);
-    if (b->cap2)
-        (void)hipStreamDestroy(b->cap2);
+    for (int k = 0; k < kMaxLanes - 1; k++) {
+        if (b->side[k])
+            (void)hipStreamDestroy(b->side[k]);
+        if (b->join[k])
+            (void)hipEventDestroy(b->join[k]);
+    }
     if (b->fork)
         (void)hipEventDestroy(b->fork);
-    if (b->join)
-        (void)hipEventDestroy(b->join);
     delete b;
 }
 

@@ -107,6 +107,24 @@ def heuristic_blk(L: np.ndarray, R: np.ndarray, thresh: float = 0.9):
     return np.where(conf >= thresh, b0 | ((b0 + 1) << 4), 7 << 4).astype(np.uint8)
 
 
+def oracle_blk(plain: np.ndarray, cover: float = 1.0) -> np.ndarray:
+    """The best windows in hindsight, from the plain D=128 disparities: per
+    tile the 32-disparity window holding the most of its pixels, kept where it
+    holds at least `cover` of them (else all blocks).  No proposal can do
+    better at the same fallback rate; it bounds what the learned path can
+    reach on a given scene."""
+    H, W = plain.shape
+    th, tw = (H + 15) // 16, (W + 15) // 16
+    hist = np.zeros((th, tw, 8))
+    blkid = plain.astype(np.int64) // 16
+    for y in range(H):
+        np.add.at(hist[y // 16], (np.arange(W) // 16, blkid[y]), 1)
+    frac = hist / hist.sum(-1, keepdims=True)
+    win = np.stack([frac[..., b] + frac[..., b + 1] for b in range(7)], -1)
+    b0, got = win.argmax(-1), win.max(-1)
+    return np.where(got >= cover, b0 | ((b0 + 1) << 4), 7 << 4).astype(np.uint8)
+
+
 def evaluate(ctx: Context, net: Cnn, seeds: Iterable[int], thresh: float = 0.9, W: int = 1920,
              H: int = 1080, reps: int = 20) -> dict:
     """Per frame: the learned path (proposal + windowed SGM) against plain D=128:
@@ -122,6 +140,7 @@ def evaluate(ctx: Context, net: Cnn, seeds: Iterable[int], thresh: float = 0.9, 
     db = torch.empty(T, dtype=torch.uint8, device="cuda")
     dh = torch.empty(T, dtype=torch.uint8, device="cuda")
     diffs, mads, falls, hdiffs, hfalls = [], [], [], [], []
+    odiffs, ofalls = [], []
     for seed in seeds:
         L, R = scene(seed, W, H)
         dl.copy_(torch.from_numpy(L))
@@ -140,6 +159,12 @@ def evaluate(ctx: Context, net: Cnn, seeds: Iterable[int], thresh: float = 0.9, 
         torch.cuda.synchronize()
         hdiffs.append(float((dw.cpu().numpy() != p).mean() * 100))
         hfalls.append(float(((hb >> 4) - (hb & 15) == 7).mean() * 100))
+        ob = oracle_blk(p).ravel()
+        dh.copy_(torch.from_numpy(ob))
+        ctx.window_dev(dl.data_ptr(), dr.data_ptr(), W, H, D, dh.data_ptr(), dw.data_ptr(), s)
+        torch.cuda.synchronize()
+        odiffs.append(float((dw.cpu().numpy() != p).mean() * 100))
+        ofalls.append(float(((ob >> 4) - (ob & 15) == 7).mean() * 100))
 
     def timed(fn):
         for _ in range(3):
@@ -170,6 +195,8 @@ def evaluate(ctx: Context, net: Cnn, seeds: Iterable[int], thresh: float = 0.9, 
             "pct_tiles_full_D": float(np.mean(falls)),
             "heuristic_pct_pixels_differ": float(np.mean(hdiffs)),
             "heuristic_pct_tiles_full_D": float(np.mean(hfalls)),
+            "hindsight_pct_pixels_differ": float(np.mean(odiffs)),
+            "hindsight_pct_tiles_full_D": float(np.mean(ofalls)),
             "ms_plain_d128": t_plain, "ms_proposal": t_prop, "ms_window_sgm": t_win,
             "ms_learned_total": t_learned, "speedup_vs_plain": t_plain / t_learned}
 
