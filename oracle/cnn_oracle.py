@@ -5,12 +5,14 @@ oracle/oracle.py for what may use the oracle).  The network has no reference
 counterpart: it is defined here and by its HIP implementation together.
 
 Definition (D = 128, frames with W and H multiples of 4):
-  features   Ld, Rd = 4x4 block sums of the left / right images (h = H/4,
-             w = W/4).  For bin k in 0..31 (full-resolution disparities
-             4k .. 4k+3):  f_k(x, y) = sum over the clamped 3x3 neighbourhood
-             (x', y') of |Ld(x', y') - Rd(x' - k, y')|, with the difference
-             taken as 4080 (= 16*255) where x' - k < 0, divided by 9*4080;
-             rounded to bf16.  Layout [h][w][32].
+  features   Ld = 4x4 block sums of the left image (h = H/4, w = W/4);
+             Rd_p = 4x4 block sums of the right image at phase p = 0..3
+             (block x covers columns 4x-p .. 4x-p+3, clamped at x = 0).
+             For bin k in 0..31 (full-resolution disparities 4k + p):
+             f_k(x, y) = min over p of the sum over the clamped 3x3
+             neighbourhood (x', y') of |Ld(x', y') - Rd_p(x' - k, y')|, the
+             difference taken as 4080 (= 16*255) where x' - k < 0, divided by
+             9*4080; rounded to bf16.  Layout [h][w][32].
   layers     6 x (3x3 conv, 32 -> 32 channels, zero padding, fp32 bias);
              layers 1-5 ReLU and bf16 output, layer 6 linear fp32 logits.
              Inputs and weights enter the products as bf16, sums in fp32.
@@ -71,25 +73,35 @@ def init_params(seed: int) -> np.ndarray:
     return np.concatenate(parts)
 
 
-def features(left, right):
-    L = np.asarray(left, np.int64)
-    R = np.asarray(right, np.int64)
-    H, W = L.shape
-    h, w = H // 4, W // 4
-    Ld = L.reshape(h, 4, w, 4).sum(axis=(1, 3))
-    Rd = R.reshape(h, 4, w, 4).sum(axis=(1, 3))
+def blocksum(img, phase=0):
+    """4x4 block sums; block x covers columns 4x - phase .. 4x - phase + 3,
+    clamped at the left edge."""
+    a = np.asarray(img, np.int64)
+    H, W = a.shape
+    cols = np.maximum(np.arange(W) - phase, 0)
+    a = a[:, cols]
+    return a.reshape(H // 4, 4, W // 4, 4).sum(axis=(1, 3))
+
+
+def features(left, right, phases=4):
+    Ld = blocksum(left)
+    h, w = Ld.shape
     f = np.empty((h, w, C), ft())
     xs = np.arange(w)
     for k in range(C):
-        diff = np.full((h, w), 4080, np.int64)
-        ok = xs >= k
-        diff[:, ok] = np.abs(Ld[:, ok] - Rd[:, xs[ok] - k])
-        pad = np.pad(diff, 1, mode="edge")
-        acc = np.zeros((h, w), np.int64)
-        for dy in range(3):
-            for dx in range(3):
-                acc += pad[dy:dy + h, dx:dx + w]
-        f[:, :, k] = acc.astype(ft()) / ft()(9 * 4080)
+        best = None
+        for ph in range(phases):
+            Rd = blocksum(right, ph)
+            diff = np.full((h, w), 4080, np.int64)
+            ok = xs >= k
+            diff[:, ok] = np.abs(Ld[:, ok] - Rd[:, xs[ok] - k])
+            pad = np.pad(diff, 1, mode="edge")
+            acc = np.zeros((h, w), np.int64)
+            for dy in range(3):
+                for dx in range(3):
+                    acc += pad[dy:dy + h, dx:dx + w]
+            best = acc if best is None else np.minimum(best, acc)
+        f[:, :, k] = best.astype(ft()) / ft()(9 * 4080)
     return bf16(f)
 
 

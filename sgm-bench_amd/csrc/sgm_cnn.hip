@@ -3,7 +3,8 @@
 //
 // Network (no reference counterpart; oracle/cnn_oracle.py is its float
 // restatement): features of a 4x-downsampled pair (32 bins of 3x3 absolute
-// differences of 4x4 block sums), 6 conv layers 3x3 32->32 (ReLU on 1-5,
+// differences of 4x4 block sums, the right image at 4 sub-block phases, the
+// best phase per bin), 6 conv layers 3x3 32->32 (ReLU on 1-5,
 // fp32 logits from 6), per 16x16-pixel tile the mean of its 4x4 low-resolution
 // logits, softmax over 32 disparity bins (4 disparities each, D = 128), the
 // 32-disparity window (two 16-blocks) with the most probability mass and that
@@ -46,41 +47,49 @@ __device__ __forceinline__ uint16_t to_bf16(float f)
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // ---- features ----
-__global__ __launch_bounds__(256) void k_blocksum(const uint8_t *__restrict__ img, int W, int H,
+// 4x4 block sums of an image at phase `ph`: block x covers full-resolution
+// columns 4x - ph .. 4x - ph + 3 (clamped at the image edges).  Phase 0 is
+// the plain 4x downsampling; phases 1-3 are the right image's sub-block
+// shifts, so that bin k can match full-resolution disparities 4k .. 4k+3.
+__global__ __launch_bounds__(256) void k_blocksum(const uint8_t *__restrict__ img, int W, int H, int phases,
                                                   int32_t *__restrict__ out)
 {
     const int w = W / 4, h = H / 4;
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= w * h)
+    if (i >= w * h * phases)
         return;
-    const int y = i / w, x = i - y * w;
+    const int ph = i / (w * h), r = i - ph * (w * h);
+    const int y = r / w, x = r - y * w;
     int s = 0;
     for (int dy = 0; dy < 4; dy++)
         for (int dx = 0; dx < 4; dx++)
-            s += img[(size_t)(4 * y + dy) * W + 4 * x + dx];
+            s += img[(size_t)(4 * y + dy) * W + max(4 * x + dx - ph, 0)];
     out[i] = s;
 }
 
-// f_k = sum_{clamped 3x3} |Ld - Rd(x'-k)| (4080 where x'-k < 0) / (9*4080), bf16.
-// A workgroup makes 32 pixels of one row, all 32 bins: the three rows of Ld
-// it needs (x0-1 .. x0+32) and of Rd (x0-32 .. x0+32) are staged in LDS;
-// thread (k, s) makes bin k of pixels x0 + 4s .. x0 + 4s + 3, so the 32
-// threads of one pixel write its 64 bytes contiguously.
+// f_k = min over the 4 phases ph of sum_{clamped 3x3} |Ld - Rd_ph(x'-k)|
+// (4080 where x'-k < 0), divided by 9*4080, bf16: bin k covers full-resolution
+// disparities 4k + ph.  A workgroup makes 32 pixels of one row, all 32 bins:
+// the three rows of Ld it needs (x0-1 .. x0+32) and of each Rd_ph (x0-32 ..
+// x0+32) are staged in LDS; thread (k, s) makes bin k of pixels x0 + 4s ..
+// x0 + 4s + 3, so the 32 threads of one pixel write its 64 bytes contiguously.
 constexpr int kFeatTX = 32;
+constexpr int kPhases = 4;
 __global__ __launch_bounds__(256) void k_features(const int32_t *__restrict__ Ld, const int32_t *__restrict__ Rd,
                                                   int w, int h, uint16_t *__restrict__ X)
 {
-    __shared__ int32_t sl[3][kFeatTX + 2], sr[3][kFeatTX + 33];
+    __shared__ int32_t sl[3][kFeatTX + 2], sr[kPhases][3][kFeatTX + 33];
     const int x0 = blockIdx.x * kFeatTX, y = blockIdx.y;
     for (int i = threadIdx.x; i < 3 * (kFeatTX + 2); i += 256) {
         const int r = i / (kFeatTX + 2), c = i - r * (kFeatTX + 2);
         const int yy = min(max(y + r - 1, 0), h - 1), xx = min(max(x0 - 1 + c, 0), w - 1);
         sl[r][c] = Ld[yy * w + xx];
     }
-    for (int i = threadIdx.x; i < 3 * (kFeatTX + 33); i += 256) {
-        const int r = i / (kFeatTX + 33), c = i - r * (kFeatTX + 33);
+    for (int i = threadIdx.x; i < kPhases * 3 * (kFeatTX + 33); i += 256) {
+        const int ph = i / (3 * (kFeatTX + 33)), rc = i - ph * (3 * (kFeatTX + 33));
+        const int r = rc / (kFeatTX + 33), c = rc - r * (kFeatTX + 33);
         const int yy = min(max(y + r - 1, 0), h - 1), xx = min(max(x0 - 32 + c, 0), w - 1);
-        sr[r][c] = Rd[yy * w + xx];
+        sr[ph][r][c] = Rd[(size_t)ph * w * h + yy * w + xx];
     }
     __syncthreads();
     const int k = threadIdx.x & 31, sub = threadIdx.x >> 5;
@@ -89,15 +98,20 @@ __global__ __launch_bounds__(256) void k_features(const int32_t *__restrict__ Ld
         const int x = x0 + 4 * sub + u;
         if (x >= w)
             break;
-        int acc = 0;
+        int best = 0x7FFFFFFF;
 #pragma unroll
-        for (int r = 0; r < 3; r++)
+        for (int ph = 0; ph < kPhases; ph++) {
+            int acc = 0;
 #pragma unroll
-            for (int dx = -1; dx <= 1; dx++) {
-                const int xx = min(max(x + dx, 0), w - 1);
-                acc += xx - k >= 0 ? abs(sl[r][xx - x0 + 1] - sr[r][xx - k - x0 + 32]) : 4080;
-            }
-        X[((size_t)y * w + x) * kC + k] = to_bf16((float)acc / (float)(9 * 4080));
+            for (int r = 0; r < 3; r++)
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    const int xx = min(max(x + dx, 0), w - 1);
+                    acc += xx - k >= 0 ? abs(sl[r][xx - x0 + 1] - sr[ph][r][xx - k - x0 + 32]) : 4080;
+                }
+            best = min(best, acc);
+        }
+        X[((size_t)y * w + x) * kC + k] = to_bf16((float)best / (float)(9 * 4080));
     }
 }
 
@@ -428,7 +442,7 @@ int cnn_resize(sgmh_cnn *c, int W, int H)
         }
     const size_t w = W / 4, h = H / 4, np = w * h, T = ((w + 3) / 4) * ((h + 3) / 4);
     CK(hipMalloc(&c->Ld, np * 4));
-    CK(hipMalloc(&c->Rd, np * 4));
+    CK(hipMalloc(&c->Rd, np * 4 * kPhases));
     for (auto &a : c->act)
         CK(hipMalloc(&a, np * kC * 2));
     CK(hipMalloc(&c->logits, np * kC * 4));
@@ -468,8 +482,8 @@ int cnn_forward(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, h
     const int w = W / 4, h = H / 4, np = w * h;
     if (!c->packed && cnn_pack(c, s))
         return -1;
-    hipLaunchKernelGGL(k_blocksum, dim3((np + 255) / 256), dim3(256), 0, s, l, W, H, c->Ld);
-    hipLaunchKernelGGL(k_blocksum, dim3((np + 255) / 256), dim3(256), 0, s, r, W, H, c->Rd);
+    hipLaunchKernelGGL(k_blocksum, dim3((np + 255) / 256), dim3(256), 0, s, l, W, H, 1, c->Ld);
+    hipLaunchKernelGGL(k_blocksum, dim3((np * kPhases + 255) / 256), dim3(256), 0, s, r, W, H, kPhases, c->Rd);
     hipLaunchKernelGGL(k_features, dim3((w + kFeatTX - 1) / kFeatTX, h), dim3(256), 0, s, c->Ld, c->Rd, w, h, c->act[0]);
     for (int i = 0; i < kLayers; i++) {
         const bool last = i == kLayers - 1;

@@ -82,20 +82,30 @@ def train(ctx: Context, net: Cnn, n_pairs: int = 2000, epochs: int = 1, lr: floa
 
 def heuristic_blk(L: np.ndarray, R: np.ndarray, thresh: float = 0.9):
     """The same proposal rule without the network: per tile, the histogram of
-    the per-pixel argmin of the network's input features (low-resolution 3x3
-    SAD over 32 bins) as the distribution.  The baseline the network must beat."""
+    the per-pixel argmin of the network's input features (the low-resolution
+    3x3 SAD over 32 bins, best of the 4 sub-block phases) as the
+    distribution.  The baseline the network must beat."""
     H, W = L.shape
-    Ld = L.astype(np.int64).reshape(H // 4, 4, W // 4, 4).sum(axis=(1, 3))
-    Rd = R.astype(np.int64).reshape(H // 4, 4, W // 4, 4).sum(axis=(1, 3))
+
+    def bsum(img, ph):
+        a = img.astype(np.int64)[:, np.maximum(np.arange(W) - ph, 0)]
+        return a.reshape(H // 4, 4, W // 4, 4).sum(axis=(1, 3))
+
+    Ld = bsum(L, 0)
+    Rds = [bsum(R, ph) for ph in range(4)]
     h, w = Ld.shape
     f = np.empty((h, w, 32), np.int64)
     xs = np.arange(w)
     for k in range(32):
-        diff = np.full((h, w), 4080, np.int64)
-        ok = xs >= k
-        diff[:, ok] = np.abs(Ld[:, ok] - Rd[:, xs[ok] - k])
-        pad = np.pad(diff, 1, mode="edge")
-        f[:, :, k] = sum(pad[dy:dy + h, dx:dx + w] for dy in range(3) for dx in range(3))
+        best = None
+        for Rd in Rds:
+            diff = np.full((h, w), 4080, np.int64)
+            ok = xs >= k
+            diff[:, ok] = np.abs(Ld[:, ok] - Rd[:, xs[ok] - k])
+            pad = np.pad(diff, 1, mode="edge")
+            acc = sum(pad[dy:dy + h, dx:dx + w] for dy in range(3) for dx in range(3))
+            best = acc if best is None else np.minimum(best, acc)
+        f[:, :, k] = best
     am = f.argmin(-1)
     th, tw = (h + 3) // 4, (w + 3) // 4
     p = np.zeros((th, tw, 32))
