@@ -699,9 +699,10 @@ struct WinArgs {
     const uint64_t *cl, *cr; // census codes, pixel i at index kLead + i
     const uint8_t *blk;      // per-tile block ranges
     int W, H, D, tiles_x;
-    uint32_t out_bytes;
+    uint32_t out_bytes;      // bytes of each direction's full volume (N*D)
     int ndirs;
-    DirTask dir[4];
+    DirTask dir[4];          // dir[k].L: full volume [N][D] (FULL / MASKED tiles)
+    uint8_t *Ln[4];          // narrow volumes [N][32]: d - 16*b0 of NARROW tiles
 };
 
 struct WinSeg {
@@ -722,7 +723,7 @@ __device__ __forceinline__ WinSeg win_seg(uint32_t rng)
 }
 
 template <int NB, bool HORIZ>
-__device__ __forceinline__ void path_window_group(const WinArgs &a, const DirTask &t, int grp)
+__device__ __forceinline__ void path_window_group(const WinArgs &a, const DirTask &t, uint8_t *narrow, int grp)
 {
     constexpr int PF = 4;
     constexpr int D = 16 * NB;
@@ -741,6 +742,7 @@ __device__ __forceinline__ void path_window_group(const WinArgs &a, const DirTas
     const uint32_t nb8 = (uint32_t)(W * H + kLead) * 8u;
     const rsrc_t rl = make_rsrc(a.cl, nb8), rr = make_rsrc(a.cr, nb8);
     const rsrc_t ro = make_rsrc(t.L, a.out_bytes);
+    const rsrc_t rn = make_rsrc(narrow, (uint32_t)(W * H) * 32u);
     const uint32_t P1 = (uint32_t)t.P1, P2 = kPlainP2, inval = kPlainInvalid;
     // Steps are counted on a tile-aligned axis q in [0, 16*nt): position
     // p = q (forward) or 16*nt-1-q (backward); real steps have 0 <= p < len.
@@ -761,7 +763,8 @@ __device__ __forceinline__ void path_window_group(const WinArgs &a, const DirTas
         const int pix = pixel(pos(q)) + kLead;
         rcl[k] = ld64(rl, (uint32_t)pix * 8u);
         // NARROW: codes of d = 16*b0 + 2l + {1, 0} in chunk 0, the other chunks
-        // repeat it; FULL/MASKED: codes of d = NB*l + NB-1-2i, NB-2-2i in chunk i
+        // repeat it; FULL/MASKED: codes of d = NB*l + NB-1-2i, NB-2-2i in chunk
+        // i.  Both modes issue the same memory instructions.
         const int lo = sg.mode == 0 ? pix - 16 * sg.b0 - 2 * l - 1 : pix - NB * l - (NB - 1);
         const int st = sg.mode == 0 ? 0 : 2;
 #pragma unroll
@@ -867,9 +870,8 @@ __device__ __forceinline__ void path_window_group(const WinArgs &a, const DirTas
                     SN[1] = umin(SN[1], kWinAbsent);
                 }
                 const bool real = q >= q0 && q < qend;
-                const uint32_t off = real ? ((uint32_t)pixel(p) * (uint32_t)D + (uint32_t)dbase) | live : kDrop;
-                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(SN[0] | (SN[1] << 8)), ro,
-                                                      dbase < D ? off : kDrop, 0, 0);
+                const uint32_t off = real ? ((uint32_t)pixel(p) * 32u + (uint32_t)(2 * l)) | live : kDrop;
+                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(SN[0] | (SN[1] << 8)), rn, off, 0, 0);
                 if (q < q0) { // before the line's first pixel (backward lines): keep the start state
                     SN[0] = 0;
                     SN[1] = 0;
@@ -924,45 +926,79 @@ __device__ __forceinline__ void path_window_group(const WinArgs &a, const DirTas
     }
 }
 
-// Sum over the directions and argmin over the pixel's active range.  One lane
-// per 16-disparity block (NB lanes per pixel, 64/NB pixels per wave): a lane
-// whose block is inactive loads nothing and bids the largest key.
+// Sum over the directions and argmin over the pixel's active range, one tile
+// (16x16 pixels, uniform range) at a time per wave.  One lane per active
+// 16-disparity block: a range of at most two blocks puts 2 lanes on a pixel
+// (32 pixels per pass) reading the dense narrow volumes, a wider one NB lanes
+// (64/NB pixels per pass) reading the full ones; lanes of inactive blocks load
+// nothing and bid the largest key.
 template <int NB>
-__device__ __forceinline__ void wta_window_batch(const uint8_t *__restrict__ L, size_t dir_stride, int ndirs,
-                                                 const uint8_t *__restrict__ blk, int tiles_x, int W, int npix,
-                                                 uint8_t *__restrict__ disp, int b0)
+__device__ __forceinline__ uint32_t wta_block_key(const uint8_t *__restrict__ src, size_t dir_stride, int ndirs,
+                                                  uint32_t d0)
 {
-    constexpr int D = 16 * NB;
-    const int lane = threadIdx.x & 63, sub = lane % NB, slot = lane / NB;
-    const int p = b0 + slot;
-    const int pc = p < npix ? p : npix - 1;
-    const int y = pc / W, x = pc - y * W;
-    const uint32_t rng = blk[(y >> 4) * tiles_x + (x >> 4)];
-    const int b1 = min((int)(rng >> 4), NB - 1), lo = min((int)(rng & 15u), b1);
-    uint32_t key = 0xFFFFFFFFu;
-    if (sub >= lo && sub <= b1) {
-        uint32_t ev[4] = {0, 0, 0, 0}, od[4] = {0, 0, 0, 0};
-        const uint8_t *src = L + (size_t)pc * D + 16 * sub;
-        for (int k = 0; k < ndirs; k++) {
-            const uint4 q = *reinterpret_cast<const uint4 *>(src + k * dir_stride);
-            const uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                ev[i] += w[i] & 0x00FF00FFu;
-                od[i] += (w[i] >> 8) & 0x00FF00FFu;
-            }
-        }
-        const uint32_t d0 = 16u * sub;
+    uint32_t ev[4] = {0, 0, 0, 0}, od[4] = {0, 0, 0, 0};
+    for (int k = 0; k < ndirs; k++) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(src + k * dir_stride);
+        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const uint32_t db = d0 + 4 * i;
-            key = umin(key, umin(((ev[i] & 0xFFFFu) << 8) | db, ((od[i] & 0xFFFFu) << 8) | (db + 1)));
-            key = umin(key, umin(((ev[i] >> 16) << 8) | (db + 2), ((od[i] >> 16) << 8) | (db + 3)));
+            ev[i] += w[i] & 0x00FF00FFu;
+            od[i] += (w[i] >> 8) & 0x00FF00FFu;
         }
     }
-    key = group_min<NB>(key);
-    if (sub == 0 && p < npix)
-        disp[p] = (uint8_t)(key & 0xFFu);
+    uint32_t key = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t db = d0 + 4 * i;
+        key = umin(key, umin(((ev[i] & 0xFFFFu) << 8) | db, ((od[i] & 0xFFFFu) << 8) | (db + 1)));
+        key = umin(key, umin(((ev[i] >> 16) << 8) | (db + 2), ((od[i] >> 16) << 8) | (db + 3)));
+    }
+    return key;
+}
+
+template <int NB>
+__device__ __forceinline__ void wta_window_tile(const uint8_t *__restrict__ L, size_t dir_stride,
+                                                const uint8_t *__restrict__ Ln, size_t nstride, int ndirs,
+                                                const uint8_t *__restrict__ blk, int tiles_x, int W, int H,
+                                                uint8_t *__restrict__ disp, int t)
+{
+    constexpr int D = 16 * NB;
+    const int lane = threadIdx.x & 63;
+    const uint32_t rng = __builtin_amdgcn_readfirstlane((uint32_t)blk[t]);
+    const int b1 = min((int)(rng >> 4), NB - 1), b0 = min((int)(rng & 15u), b1);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    if (b1 - b0 <= 1) {
+        const int b = b0 + (lane & 1);
+        const bool act = b <= b1;
+#pragma unroll 2
+        for (int it = 0; it < 8; it++) {
+            const int r = it * 32 + (lane >> 1), y = ty * 16 + (r >> 4), x = tx * 16 + (r & 15);
+            const bool valid = y < H && x < W;
+            const size_t p = valid ? (size_t)y * W + x : 0;
+            uint32_t key = 0xFFFFFFFFu;
+            if (act && valid)
+                key = wta_block_key<NB>(Ln + p * 32 + 16 * (lane & 1), nstride, ndirs, 16u * b);
+            key = umin(key, dpp<kQuadSwap1>(key));
+            if (!(lane & 1) && valid)
+                disp[p] = (uint8_t)(key & 0xFFu);
+        }
+    } else {
+        constexpr int PPW = 64 / NB;
+        const int sub = lane % NB;
+        const bool act = sub >= b0 && sub <= b1;
+#pragma unroll 2
+        for (int it = 0; it < 256 / PPW; it++) {
+            const int r = it * PPW + lane / NB, y = ty * 16 + (r >> 4), x = tx * 16 + (r & 15);
+            const bool valid = y < H && x < W;
+            const size_t p = valid ? (size_t)y * W + x : 0;
+            uint32_t key = 0xFFFFFFFFu;
+            if (act && valid)
+                key = wta_block_key<NB>(L + p * D + 16 * sub, dir_stride, ndirs, 16u * sub);
+            key = group_min<NB>(key);
+            if (sub == 0 && valid)
+                disp[p] = (uint8_t)(key & 0xFFu);
+        }
+    }
 }
 
 } // namespace sgmk

@@ -124,21 +124,20 @@ __global__ __launch_bounds__(64) void k_path_window(WinArgs a)
     const DirTask t = a.dir[k];
     const int grp = (int)blockIdx.x - t.first_block;
     if (t.dy == 0)
-        path_window_group<NB, true>(a, t, grp);
+        path_window_group<NB, true>(a, t, a.Ln[k], grp);
     else
-        path_window_group<NB, false>(a, t, grp);
+        path_window_group<NB, false>(a, t, a.Ln[k], grp);
 }
 
 template <int NB>
-__global__ __launch_bounds__(256) void k_wta_window(const uint8_t *__restrict__ L, size_t dir_stride, int ndirs,
-                                                    const uint8_t *__restrict__ blk, int tiles_x, int W, int npix,
-                                                    uint8_t *__restrict__ disp)
+__global__ __launch_bounds__(256) void k_wta_window(const uint8_t *__restrict__ L, size_t dir_stride,
+                                                    const uint8_t *__restrict__ Ln, size_t nstride, int ndirs,
+                                                    const uint8_t *__restrict__ blk, int tiles_x, int ntiles, int W,
+                                                    int H, uint8_t *__restrict__ disp)
 {
-    constexpr int PPW = 64 / NB;
     const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
-    const int nwaves = (int)gridDim.x * 4;
-    for (int b0 = wave * PPW; b0 < npix; b0 += nwaves * PPW)
-        wta_window_batch<NB>(L, dir_stride, ndirs, blk, tiles_x, W, npix, disp, b0);
+    for (int t = wave; t < ntiles; t += (int)gridDim.x * 4)
+        wta_window_tile<NB>(L, dir_stride, Ln, nstride, ndirs, blk, tiles_x, W, H, disp, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,11 +343,12 @@ static int window_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const 
                           int D, const uint8_t *blk, uint8_t *disp)
 {
     const size_t N = (size_t)W * H;
-    REQUIRE(N * D < (1ull << 31), "sgmh_window: %dx%d D=%d exceeds the 2 GiB per-volume limit", W, H, D);
-    if (ensure_codes(ctx->cl, N) || ensure_codes(ctx->cr, N) || ensure(ctx->vol, N * D * 4))
+    REQUIRE(N * D < (1ull << 31) && N * 32 < (1ull << 31), "sgmh_window: %dx%d D=%d exceeds the 2 GiB per-volume limit", W, H, D);
+    if (ensure_codes(ctx->cl, N) || ensure_codes(ctx->cr, N) || ensure(ctx->vol, N * (D + 32) * 4))
         return -1;
     uint64_t *cl = (uint64_t *)ctx->cl.p, *cr = (uint64_t *)ctx->cr.p;
     uint8_t *vol = (uint8_t *)ctx->vol.p;
+    uint8_t *nvol = vol + N * D * 4; // narrow volumes, [N][32] per direction
     {
         PhaseScope ps(ctx, s, PH_CENSUS);
         if (launch_census(s, l, cl + kLead, W, H, W) || launch_census(s, r, cr + kLead, W, H, W))
@@ -373,6 +373,7 @@ static int window_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const 
         t.nlines = t.dy == 0 ? H : W;
         t.first_block = blocks;
         t.L = vol + (size_t)k * N * D;
+        a.Ln[k] = nvol + (size_t)k * N * 32;
         blocks += (t.nlines + kLinesPerWave - 1) / kLinesPerWave;
     }
     {
@@ -387,11 +388,12 @@ static int window_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const 
     }
     {
         PhaseScope ps(ctx, s, PH_ARGMIN);
-        const int wb = (int)std::min<size_t>((N + (size_t)(64 / (D / 16)) * 4 - 1) / ((size_t)(64 / (D / 16)) * 4), 16384);
+        const int ntiles = a.tiles_x * ((H + 15) / 16);
+        const int wb = (ntiles + 3) / 4;
         switch (D / 16) {
-        case 2: hipLaunchKernelGGL(k_wta_window<2>, dim3(wb), dim3(256), 0, s, vol, N * D, 4, blk, a.tiles_x, W, (int)N, disp); break;
-        case 4: hipLaunchKernelGGL(k_wta_window<4>, dim3(wb), dim3(256), 0, s, vol, N * D, 4, blk, a.tiles_x, W, (int)N, disp); break;
-        default: hipLaunchKernelGGL(k_wta_window<8>, dim3(wb), dim3(256), 0, s, vol, N * D, 4, blk, a.tiles_x, W, (int)N, disp); break;
+        case 2: hipLaunchKernelGGL(k_wta_window<2>, dim3(wb), dim3(256), 0, s, vol, N * D, nvol, N * 32, 4, blk, a.tiles_x, ntiles, W, H, disp); break;
+        case 4: hipLaunchKernelGGL(k_wta_window<4>, dim3(wb), dim3(256), 0, s, vol, N * D, nvol, N * 32, 4, blk, a.tiles_x, ntiles, W, H, disp); break;
+        default: hipLaunchKernelGGL(k_wta_window<8>, dim3(wb), dim3(256), 0, s, vol, N * D, nvol, N * 32, 4, blk, a.tiles_x, ntiles, W, H, disp); break;
         }
         CK(hipGetLastError());
     }
