@@ -14,15 +14,19 @@ Definition (D = 128, frames with W and H multiples of 4):
              difference taken as 4080 (= 16*255) where x' - k < 0, divided by
              9*4080; rounded to bf16.  Layout [h][w][32].
   layers     6 x (3x3 conv, 32 -> 32 channels, zero padding, fp32 bias);
-             layers 1-5 ReLU and bf16 output, layer 6 linear fp32 logits.
+             layers 1-5 ReLU and bf16 output, layer 6 linear fp32 logits,
+             to which the fixed residual -SKIP * features is added (a low
+             matching cost is a high logit before the network's correction).
              Inputs and weights enter the products as bf16, sums in fp32.
-  head       16x16-pixel tile t = 4x4 low-resolution pixels (fewer at the
-             bottom / right edge): z_t = mean of their logits, p_t =
-             softmax(z_t).  Window start b0 (16-disparity blocks) maximises
-             conf = sum of p over bins 4*b0 .. 4*b0+7; below the confidence
-             threshold the tile falls back to all blocks.
-  loss       cross-entropy -sum_k q_tk log p_tk, averaged over tiles, with q_t
-             the histogram of the tile's target disparities in bins of 4.
+  head       per low-resolution pixel p = softmax(logits); per 16x16-pixel
+             tile (4x4 low-resolution pixels, fewer at the bottom / right
+             edge) P_t = the mean of its pixels' p.  Window start b0
+             (16-disparity blocks) maximises conf = sum of P_t over bins
+             4*b0 .. 4*b0+7; below the confidence threshold the tile falls
+             back to all blocks.
+  loss       per-pixel cross-entropy -sum_k q_k log p_k, averaged over the
+             low-resolution pixels, with q the histogram of the target
+             disparities of the pixel's 4x4 block in bins of 4.
 Parameters are stored flat in fp32: for each layer W[co][ci][tap] (tap =
 3*ky + kx) then b[co].
 """
@@ -31,6 +35,7 @@ from __future__ import annotations
 import numpy as np
 
 C = 32          # channels / bins
+SKIP = 32.0     # logits = last conv - SKIP * features (a fixed residual from the matching costs)
 ROUND = True    # bf16 roundings on; off only for finite-difference checks
 LAYERS = 6
 NPARAM = LAYERS * (C * C * 9 + C)
@@ -128,20 +133,8 @@ def forward(theta, x0):
             x = bf16(np.maximum(z, 0))
             acts.append(x)
         else:
-            x = z
+            x = z - ft()(SKIP) * x0
     return acts, x   # acts[i] = input of layer i; x = logits
-
-
-def tile_logits(logits):
-    h, w, _ = logits.shape
-    th, tw = (h + 3) // 4, (w + 3) // 4
-    z = np.zeros((th, tw, C), ft())
-    n = np.zeros((th, tw, 1), ft())
-    for y in range(h):
-        for x in range(w):
-            z[y // 4, x // 4] += logits[y, x]
-            n[y // 4, x // 4] += 1
-    return z / n, n
 
 
 def softmax(z):
@@ -149,34 +142,47 @@ def softmax(z):
     return e / e.sum(axis=-1, keepdims=True)
 
 
-def targets(disp, H, W):
-    """Per-tile histogram of target disparities in bins of 4 (D = 128)."""
-    th, tw = (H + 15) // 16, (W + 15) // 16
-    q = np.zeros((th, tw, C), ft())
-    d = np.asarray(disp, np.int64) // 4
-    for y in range(H):
-        np.add.at(q[y // 16], (np.arange(W) // 16, d[y]), 1)
-    return q / q.sum(axis=-1, keepdims=True)
+def tile_probs(logits):
+    """Per 16x16 tile (4x4 low-resolution pixels, fewer at the edges): the
+    mean of its pixels' softmax distributions over the 32 bins."""
+    p = softmax(logits)
+    h, w, _ = p.shape
+    th, tw = (h + 3) // 4, (w + 3) // 4
+    P = np.zeros((th, tw, C), ft())
+    n = np.zeros((th, tw, 1), ft())
+    for y in range(h):
+        for x in range(w):
+            P[y // 4, x // 4] += p[y, x]
+            n[y // 4, x // 4] += 1
+    return P / n
+
+
+def targets(disp):
+    """Per low-resolution pixel: the histogram of the target disparities of
+    its 4x4 full-resolution block, in bins of 4 (D = 128)."""
+    d = np.minimum(np.asarray(disp, np.int64) // 4, C - 1)
+    H, W = d.shape
+    h, w = H // 4, W // 4
+    q = np.zeros((h, w, C), ft())
+    blk = d[:4 * h, :4 * w].reshape(h, 4, w, 4).transpose(0, 2, 1, 3).reshape(h, w, 16)
+    for k in range(C):
+        q[..., k] = (blk == k).sum(-1)
+    return q / 16
 
 
 def loss_and_grads(theta, left, right, disp):
-    """Mean tile cross-entropy and its gradient w.r.t. theta (fp32 backprop;
-    the bf16 roundings of the forward are treated as identity)."""
+    """Mean per-pixel cross-entropy against the block histograms and its
+    gradient w.r.t. theta (fp32 backprop; the bf16 roundings of the forward are
+    treated as identity)."""
     theta = np.asarray(theta, ft())
-    H, W = np.asarray(left).shape
     x0 = features(left, right)
     acts, logits = forward(theta, x0)
-    z, n = tile_logits(logits)
-    p = softmax(z)
-    q = targets(disp, H, W)
-    T = z.shape[0] * z.shape[1]
-    loss = float(-(q * np.log(np.maximum(p, 1e-30))).sum() / T)
-    dz = (p - q) / T
+    p = softmax(logits)
+    q = targets(disp)
     h, w, _ = logits.shape
-    dy = np.empty_like(logits)
-    for y in range(h):
-        for x in range(w):
-            dy[y, x] = dz[y // 4, x // 4] / n[y // 4, x // 4]
+    npix = h * w
+    loss = float(-(q * np.log(np.maximum(p, 1e-30))).sum() / npix)
+    dy = ((p - q) / npix).astype(ft())
     params = split_params(theta)
     grads = [None] * LAYERS
     for i in range(LAYERS - 1, -1, -1):
@@ -207,8 +213,7 @@ def loss_and_grads(theta, left, right, disp):
 def propose(theta, left, right, thresh=0.9):
     """Per-tile block ranges (b0 | b1 << 4) and confidences, D = 128."""
     _, logits = forward(theta, features(left, right))
-    z, _ = tile_logits(logits)
-    p = softmax(z)
+    p = tile_probs(logits)
     win = np.stack([p[..., 4 * b:4 * b + 8].sum(-1) for b in range(7)], -1)
     b0 = win.argmax(-1)
     conf = win.max(-1)

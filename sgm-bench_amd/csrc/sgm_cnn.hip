@@ -4,19 +4,22 @@
 // Network (no reference counterpart; oracle/cnn_oracle.py is its float
 // restatement): features of a 4x-downsampled pair (32 bins of 3x3 absolute
 // differences of 4x4 block sums, the right image at 4 sub-block phases, the
-// best phase per bin), 6 conv layers 3x3 32->32 (ReLU on 1-5,
-// fp32 logits from 6), per 16x16-pixel tile the mean of its 4x4 low-resolution
-// logits, softmax over 32 disparity bins (4 disparities each, D = 128), the
-// 32-disparity window (two 16-blocks) with the most probability mass and that
-// mass as the confidence.  Training minimises the tile cross-entropy against
-// the histogram of the plain full-D SGM disparities of the tile.
+// best phase per bin), 6 conv layers 3x3 32->32 (ReLU on 1-5, fp32 logits
+// from 6 plus the fixed residual -32 * features: a low matching cost is a high
+// logit before the network's correction), a softmax over the 32 disparity
+// bins (4 disparities each, D = 128) per low-resolution pixel, and per
+// 16x16-pixel tile the mean of its 4x4 pixels' distributions: the 32-disparity
+// window (two 16-blocks) with the most mass wins and that mass is the
+// confidence.  Training minimises the per-pixel cross-entropy against the
+// histogram of the plain full-D SGM disparities of the pixel's 4x4
+// full-resolution block.
 //
 // Convolutions are implicit GEMMs on v_mfma_f32_16x16x16bf16_1k: a wave owns
-// 16 consecutive pixels of a row (M) x 32 output channels (two 16-wide N
+// 16 consecutive pixels of a row x 32 output channels (two 16-channel
 // tiles); K = 9 taps x 32 input channels = 18 steps of 16.  The layer's packed
-// bf16 weights (the B operands of all 18 steps) stay in registers for the
-// wave's whole pixel loop; activations are NHWC bf16, 64 bytes per pixel, so
-// one A operand is one 8-byte load per lane.
+// bf16 weights (one operand of all 18 steps) stay in registers for the
+// wave's whole pixel loop; activations are NHWC bf16, 64 bytes per pixel, and
+// a pixel operand is one 8-byte LDS read per lane from the staged patch.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -34,6 +37,7 @@ constexpr int kWPer = kC * kC * 9; // weights per layer
 constexpr int kPPer = kWPer + kC;  // parameters per layer (weights + bias)
 constexpr int kNParam = kLayers * kPPer;
 constexpr int kDisp = 128;
+constexpr float kSkip = 32.f; // logits = last conv - kSkip * features
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -93,114 +97,167 @@ __global__ __launch_bounds__(256) void k_features(const int32_t *__restrict__ Ld
     }
     __syncthreads();
     const int k = threadIdx.x & 31, sub = threadIdx.x >> 5;
+    const int xb = x0 + 4 * sub;
+    // the six clamped columns xb-1 .. xb+4 that the thread's 4 pixels' 3x3
+    // windows touch; each column's difference is formed once per row and phase
+    int cx[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++)
+        cx[j] = min(max(xb - 1 + j, 0), w - 1);
+    int best[4] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF};
+#pragma unroll
+    for (int ph = 0; ph < kPhases; ph++) {
+        int acc[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            int dcol[6];
+#pragma unroll
+            for (int j = 0; j < 6; j++)
+                dcol[j] = cx[j] - k >= 0 ? abs(sl[r][cx[j] - x0 + 1] - sr[ph][r][cx[j] - k - x0 + 32]) : 4080;
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                acc[u] += dcol[u] + dcol[u + 1] + dcol[u + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            best[u] = min(best[u], acc[u]);
+    }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-        const int x = x0 + 4 * sub + u;
-        if (x >= w)
-            break;
-        int best = 0x7FFFFFFF;
-#pragma unroll
-        for (int ph = 0; ph < kPhases; ph++) {
-            int acc = 0;
-#pragma unroll
-            for (int r = 0; r < 3; r++)
-#pragma unroll
-                for (int dx = -1; dx <= 1; dx++) {
-                    const int xx = min(max(x + dx, 0), w - 1);
-                    acc += xx - k >= 0 ? abs(sl[r][xx - x0 + 1] - sr[ph][r][xx - k - x0 + 32]) : 4080;
-                }
-            best = min(best, acc);
-        }
-        X[((size_t)y * w + x) * kC + k] = to_bf16((float)best / (float)(9 * 4080));
+        const int x = xb + u;
+        if (x < w)
+            X[((size_t)y * w + x) * kC + k] = to_bf16((float)best[u] / (float)(9 * 4080));
     }
 }
 
 // ---- convolution (implicit GEMM on MFMA) ----
 // mode 0: Y = relu(conv + b) as bf16      (hidden layers)
-// mode 1: Y = conv + b as fp32            (logits)
+// mode 1: Y = conv + b - kSkip * features as fp32 (logits; `mask` = features)
 // mode 2: Y = conv * (mask > 0) as bf16    (backward data through a ReLU)
 // Wp: packed bf16 B operands, [co][k] with k = tap*32 + ci, 32 x 288.
-// A workgroup of 4 waves stages the layer's 18 KB of B operands in LDS once;
-// each wave keeps its copy in registers and walks 16-pixel row tiles, loading
-// the next tile's 18 A operands while the MFMAs of the current one run.
-// Out-of-image taps read through an out-of-range buffer offset (zeros), so
-// the tile loop has no divergent loads.
+// A workgroup of 4 waves makes a block of 4 rows x 16 pixels x 32 channels.
+// Its input patch -- 6 rows x 18 pixels x 32 channels, out-of-image pixels as
+// zeros -- is staged in LDS by 16-byte loads (two per thread), double
+// buffered: the next block's patch is in flight while the MFMAs of this one
+// run.  Wave w reads its 18 A operands (row w + ky, pixel lr + kx, 4 channels)
+// from the patch; the layer's B operands stay in registers.
 constexpr int kConvWaves = 4;
+constexpr int kPatchRows = kConvWaves + 2, kPatchCols = 18;
+constexpr int kPatchPx = 72;                          // bytes per staged pixel (64 + 8 against bank conflicts)
+constexpr int kPatchBytes = kPatchRows * kPatchCols * kPatchPx;
+constexpr int kPatchChunks = kPatchRows * kPatchCols * 4; // 16-byte chunks (4 per pixel)
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t *__restrict__ X, int w, int h,
                                               const uint16_t *__restrict__ Wp, const float *__restrict__ bias,
                                               const uint16_t *__restrict__ mask, void *__restrict__ Y)
 {
+    __shared__ __align__(16) uint8_t patch[2][kPatchBytes];
     __shared__ uint2 wlds[kC * 288 / 4]; // 32 x 288 bf16
     for (int i = threadIdx.x; i < kC * 288 / 4; i += 256)
         wlds[i] = reinterpret_cast<const uint2 *>(Wp)[i];
-    __syncthreads();
     const int l = threadIdx.x & 63, lr = l & 15, lq = l >> 4, wv = threadIdx.x >> 6;
+    const int bx = (w + 15) / 16, by = (h + kConvWaves - 1) / kConvWaves, nblk = bx * by;
+    const sgmk::rsrc_t rx = sgmk::make_rsrc(X, (uint32_t)((size_t)w * h * kC * 2));
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    // this thread's (up to) two 16-byte chunks of a block's patch
+    v4u stage[2];
+    auto fetch = [&](int bk) {
+        const int y0 = (bk / bx) * kConvWaves - 1, x0 = (bk % bx) * 16 - 1;
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int c = threadIdx.x + 256 * u;
+            const int px = c >> 2, q = c & 3, r = px / kPatchCols, col = px - r * kPatchCols;
+            const int yy = y0 + r, xx = x0 + col;
+            const bool in = bk < nblk && c < kPatchChunks && yy >= 0 && yy < h && xx >= 0 && xx < w;
+            const uint32_t off = (uint32_t)(((yy * w + xx) * kC + 8 * q) * 2);
+            stage[u] = __builtin_amdgcn_raw_buffer_load_b128(rx, in ? off : sgmk::kDrop, 0, 0);
+        }
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int c = threadIdx.x + 256 * u;
+            if (c < kPatchChunks)
+                *reinterpret_cast<v4u *>(&patch[buf][(c >> 2) * kPatchPx + 16 * (c & 3)]) = stage[u];
+        }
+    };
+    float bch[2][4] = {};
+    if constexpr (MODE != 2) {
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                bch[nt][i] = bias[16 * nt + 4 * lq + i];
+    }
+    int bk = blockIdx.x;
+    fetch(bk);
+    commit(0);
+    __syncthreads(); // weights and the first patch staged
     s16x4 B[18][2];
 #pragma unroll
     for (int kb = 0; kb < 18; kb++)
 #pragma unroll
         for (int nt = 0; nt < 2; nt++)
             B[kb][nt] = __builtin_bit_cast(s16x4, wlds[((16 * nt + lr) * 288 + 16 * kb + 4 * lq) / 4]);
-    const int tiles_x = (w + 15) / 16, ntiles = tiles_x * h;
-    const sgmk::rsrc_t rx = sgmk::make_rsrc(X, (uint32_t)((size_t)w * h * kC * 2));
-    auto load_tile = [&](int t, uint2 (&A)[18]) {
-        const int y = t / tiles_x, x0 = (t - y * tiles_x) * 16;
-#pragma unroll
-        for (int kb = 0; kb < 18; kb++) {
-            const int tap = kb >> 1, ky = tap / 3, kx = tap % 3, ci = (kb & 1) * 16 + 4 * lq;
-            const int xx = x0 + lr + kx - 1, yy = y + ky - 1;
-            const bool in = t < ntiles && xx >= 0 && xx < w && yy >= 0 && yy < h;
-            const uint32_t off = (uint32_t)(((yy * w + xx) * kC + ci) * 2);
-            const auto v = __builtin_amdgcn_raw_buffer_load_b64(rx, in ? off : sgmk::kDrop, 0, 0);
-            A[kb] = make_uint2(v[0], v[1]);
-        }
-    };
-    float bch[2] = {0.f, 0.f};
-    if constexpr (MODE != 2) {
-        bch[0] = bias[lr];
-        bch[1] = bias[16 + lr];
-    }
-    const int stride = gridDim.x * kConvWaves;
-    int t = blockIdx.x * kConvWaves + wv;
-    uint2 A[18], An[18];
-    load_tile(t, A);
-    for (; t < ntiles; t += stride) {
-        load_tile(t + stride, An);
-        const int y = t / tiles_x, x0 = (t - y * tiles_x) * 16;
+    int buf = 0;
+    for (; bk < nblk; bk += gridDim.x) {
+        fetch(bk + gridDim.x);
+        const uint8_t *pt = patch[buf];
         f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kb = 0; kb < 18; kb++) {
-            const s16x4 a = __builtin_bit_cast(s16x4, A[kb]);
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[kb][0], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, B[kb][1], acc1, 0, 0, 0);
+            const int tap = kb >> 1, ky = tap / 3, kx = tap % 3;
+            const int ci = (kb & 1) * 16 + 4 * lq;
+            const uint2 v = *reinterpret_cast<const uint2 *>(pt + ((wv + ky) * kPatchCols + lr + kx) * kPatchPx + 2 * ci);
+            const s16x4 a = __builtin_bit_cast(s16x4, v);
+            // D^T = W X^T: the weights are the A operand, the pixels the B
+            // operand, so each lane ends with 4 consecutive channels of one pixel
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(B[kb][0], a, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(B[kb][1], a, acc1, 0, 0, 0);
         }
-        // lane holds D[pixel 4*lq + i][channel lr] of each 16-channel tile
+        // lane holds D[channel 16*nt + 4*lq + i][pixel lr]
+        const int y = (bk / bx) * kConvWaves + wv, x = (bk % bx) * 16 + lr;
+        if (y < h && x < w) {
+            const size_t o = ((size_t)y * w + x) * kC;
 #pragma unroll
-        for (int nt = 0; nt < 2; nt++) {
-            const f32x4 acc = nt ? acc1 : acc0;
-            const int ch = 16 * nt + lr;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int x = x0 + 4 * lq + i;
-                if (x >= w)
-                    continue;
-                const size_t o = ((size_t)y * w + x) * kC + ch;
-                float v = acc[i];
+            for (int nt = 0; nt < 2; nt++) {
+                const f32x4 acc = nt ? acc1 : acc0;
+                const int c0 = 16 * nt + 4 * lq;
                 if constexpr (MODE == 0) {
-                    v = fmaxf(v + bch[nt], 0.f);
-                    reinterpret_cast<uint16_t *>(Y)[o] = to_bf16(v);
+                    uint32_t pk[2];
+#pragma unroll
+                    for (int i = 0; i < 4; i += 2)
+                        pk[i / 2] = (uint32_t)to_bf16(fmaxf(acc[i] + bch[nt][i], 0.f)) |
+                                    ((uint32_t)to_bf16(fmaxf(acc[i + 1] + bch[nt][i + 1], 0.f)) << 16);
+                    *reinterpret_cast<uint2 *>(reinterpret_cast<uint16_t *>(Y) + o + c0) = make_uint2(pk[0], pk[1]);
                 } else if constexpr (MODE == 1) {
-                    reinterpret_cast<float *>(Y)[o] = v + bch[nt];
+                    // logits: the fixed residual from the matching costs (passed as `mask`)
+                    const uint2 fm = *reinterpret_cast<const uint2 *>(mask + o + c0);
+                    const uint16_t f[4] = {(uint16_t)fm.x, (uint16_t)(fm.x >> 16), (uint16_t)fm.y,
+                                           (uint16_t)(fm.y >> 16)};
+                    float4 r;
+                    r.x = acc[0] + bch[nt][0] - kSkip * from_bf16(f[0]);
+                    r.y = acc[1] + bch[nt][1] - kSkip * from_bf16(f[1]);
+                    r.z = acc[2] + bch[nt][2] - kSkip * from_bf16(f[2]);
+                    r.w = acc[3] + bch[nt][3] - kSkip * from_bf16(f[3]);
+                    *reinterpret_cast<float4 *>(reinterpret_cast<float *>(Y) + o + c0) = r;
                 } else {
-                    v = from_bf16(mask[o]) > 0.f ? v : 0.f;
-                    reinterpret_cast<uint16_t *>(Y)[o] = to_bf16(v);
+                    const uint2 mm = *reinterpret_cast<const uint2 *>(mask + o + c0);
+                    const uint16_t m[4] = {(uint16_t)mm.x, (uint16_t)(mm.x >> 16), (uint16_t)mm.y,
+                                           (uint16_t)(mm.y >> 16)};
+                    uint32_t pk[2];
+#pragma unroll
+                    for (int i = 0; i < 4; i += 2)
+                        pk[i / 2] = (uint32_t)to_bf16(from_bf16(m[i]) > 0.f ? acc[i] : 0.f) |
+                                    ((uint32_t)to_bf16(from_bf16(m[i + 1]) > 0.f ? acc[i + 1] : 0.f) << 16);
+                    *reinterpret_cast<uint2 *>(reinterpret_cast<uint16_t *>(Y) + o + c0) = make_uint2(pk[0], pk[1]);
                 }
             }
         }
-#pragma unroll
-        for (int kb = 0; kb < 18; kb++)
-            A[kb] = An[kb];
+        buf ^= 1;
+        commit(buf); // the next block's patch, into the buffer not read above
+        __syncthreads();
     }
 }
 
@@ -282,22 +339,39 @@ __global__ __launch_bounds__(256) void k_bias_grad(const uint16_t *__restrict__ 
     atomicAdd(&db[c], s);
 }
 
-// ---- tile head ----
-// hist[t][k]: counts of target disparities of tile t in bin k
-__global__ __launch_bounds__(256) void k_tile_hist(const uint8_t *__restrict__ disp, int W, int H, int *__restrict__ hist)
+// ---- head ----
+// hist[p][k]: how many of low-resolution pixel p's 16 target disparities (its
+// 4x4 full-resolution block) fall in bin k
+__global__ __launch_bounds__(256) void k_pixel_hist(const uint8_t *__restrict__ disp, int W, int H,
+                                                    uint8_t *__restrict__ hist)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= W * H)
+    const int w = W / 4, h = H / 4;
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= w * h)
         return;
-    const int y = i / W, x = i - y * W, tx = (W + 15) / 16;
-    atomicAdd(&hist[((y >> 4) * tx + (x >> 4)) * kC + min(disp[i] >> 2, kC - 1)], 1);
+    const int y = p / w, x = p - y * w;
+    uint32_t cnt[kC / 4] = {}; // four 8-bit counters per word
+    for (int r = 0; r < 4; r++) {
+        const uint32_t v = *reinterpret_cast<const uint32_t *>(disp + (size_t)(4 * y + r) * W + 4 * x);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t bin = min((v >> (8 * i) & 0xFFu) >> 2, (uint32_t)kC - 1);
+#pragma unroll
+            for (int q = 0; q < kC / 4; q++)
+                cnt[q] += (bin >> 2) == (uint32_t)q ? 1u << (8 * (bin & 3)) : 0u;
+        }
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(hist + (size_t)p * kC);
+    dst[0] = make_uint4(cnt[0], cnt[1], cnt[2], cnt[3]);
+    dst[1] = make_uint4(cnt[4], cnt[5], cnt[6], cnt[7]);
 }
 
-// One 32-lane half-wave per tile (lane = bin).  Inference writes blk / conf;
-// training (hist != nullptr) also writes dlogits for the tile's pixels and
-// adds the tile's cross-entropy / T to *loss.
+// One 32-lane half-wave per 16x16 tile (lane = bin): the tile distribution is
+// the mean of its low-resolution pixels' softmax.  Inference writes blk /
+// conf; training (hist != nullptr) also writes each pixel's dlogits = (p - q) /
+// npix and adds the pixels' cross-entropy / npix to *loss.
 __global__ __launch_bounds__(64) void k_tile_head(const float *__restrict__ logits, int w, int h,
-                                                  const int *__restrict__ hist, float thresh,
+                                                  const uint8_t *__restrict__ hist, float thresh,
                                                   uint8_t *__restrict__ blk, float *__restrict__ conf,
                                                   uint16_t *__restrict__ dlogits, float *__restrict__ loss)
 {
@@ -307,29 +381,37 @@ __global__ __launch_bounds__(64) void k_tile_head(const float *__restrict__ logi
     if (t >= T)
         return;
     const int ty = t / tw, tx = t - ty * tw;
-    float z = 0.f;
+    const float inv_np = 1.f / (float)(w * h);
+    float P = 0.f, ce = 0.f;
     int n = 0;
     for (int y = 4 * ty; y < min(4 * ty + 4, h); y++)
         for (int x = 4 * tx; x < min(4 * tx + 4, w); x++) {
-            z += logits[((size_t)y * w + x) * kC + k];
+            const size_t o = ((size_t)y * w + x) * kC + k;
+            const float z = logits[o];
+            float mx = z;
+            for (int s = 16; s; s >>= 1)
+                mx = fmaxf(mx, __shfl_xor(mx, s, 32));
+            const float e = expf(z - mx);
+            float se = e;
+            for (int s = 16; s; s >>= 1)
+                se += __shfl_xor(se, s, 32);
+            const float p = e / se;
+            P += p;
             n++;
+            if (hist) {
+                const float q = (float)hist[o] * (1.f / 16.f);
+                ce -= q * logf(fmaxf(p, 1e-30f));
+                dlogits[o] = to_bf16((p - q) * inv_np);
+            }
         }
-    z /= (float)n;
-    float mx = z;
-    for (int o = 16; o; o >>= 1)
-        mx = fmaxf(mx, __shfl_xor(mx, o, 32));
-    const float e = expf(z - mx);
-    float se = e;
-    for (int o = 16; o; o >>= 1)
-        se += __shfl_xor(se, o, 32);
-    const float p = e / se;
+    P /= (float)n;
     // probability mass of the window of bins [4 b0, 4 b0 + 8)
     float best = -1.f;
     int bb = 0;
     for (int b0 = 0; b0 < kDisp / 16 - 1; b0++) {
-        float m = (k >= 4 * b0 && k < 4 * b0 + 8) ? p : 0.f;
-        for (int o = 16; o; o >>= 1)
-            m += __shfl_xor(m, o, 32);
+        float m = (k >= 4 * b0 && k < 4 * b0 + 8) ? P : 0.f;
+        for (int s = 16; s; s >>= 1)
+            m += __shfl_xor(m, s, 32);
         if (m > best) {
             best = m;
             bb = b0;
@@ -340,19 +422,10 @@ __global__ __launch_bounds__(64) void k_tile_head(const float *__restrict__ logi
         blk[t] = best >= thresh ? (uint8_t)(bb | ((bb + 1) << 4)) : (uint8_t)((kDisp / 16 - 1) << 4);
     }
     if (hist) {
-        int c = hist[t * kC + k], tot = c;
-        for (int o = 16; o; o >>= 1)
-            tot += __shfl_xor(tot, o, 32);
-        const float q = (float)c / (float)tot;
-        float ce = -q * logf(fmaxf(p, 1e-30f));
-        for (int o = 16; o; o >>= 1)
-            ce += __shfl_xor(ce, o, 32);
+        for (int s = 16; s; s >>= 1)
+            ce += __shfl_xor(ce, s, 32);
         if (k == 0)
-            atomicAdd(loss, ce / (float)T);
-        const uint16_t g = to_bf16((p - q) / (float)T / (float)n);
-        for (int y = 4 * ty; y < min(4 * ty + 4, h); y++)
-            for (int x = 4 * tx; x < min(4 * tx + 4, w); x++)
-                dlogits[((size_t)y * w + x) * kC + k] = g;
+            atomicAdd(loss, ce * inv_np);
     }
 }
 
@@ -394,13 +467,6 @@ __global__ __launch_bounds__(256) void k_zero(float *p, int n)
         p[i] = 0.f;
 }
 
-__global__ __launch_bounds__(256) void k_zero_i(int *p, int n)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n)
-        p[i] = 0;
-}
-
 } // namespace
 
 struct sgmh_cnn {
@@ -415,7 +481,7 @@ struct sgmh_cnn {
     uint16_t *act[kLayers] = {};   // act[i] = input of layer i (bf16 NHWC)
     float *logits = nullptr;
     uint16_t *g0 = nullptr, *g1 = nullptr; // gradient ping-pong (bf16)
-    int *hist = nullptr;
+    uint8_t *hist = nullptr; // per low-resolution pixel target histograms
     float *loss = nullptr;
     uint8_t *blk = nullptr;
     float *conf = nullptr;
@@ -448,7 +514,7 @@ int cnn_resize(sgmh_cnn *c, int W, int H)
     CK(hipMalloc(&c->logits, np * kC * 4));
     CK(hipMalloc(&c->g0, np * kC * 2));
     CK(hipMalloc(&c->g1, np * kC * 2));
-    CK(hipMalloc(&c->hist, T * kC * 4));
+    CK(hipMalloc(&c->hist, np * kC));
     CK(hipMalloc(&c->loss, 64));
     CK(hipMalloc(&c->blk, T));
     CK(hipMalloc(&c->conf, T * 4));
@@ -474,7 +540,7 @@ int cnn_pack(sgmh_cnn *c, hipStream_t s)
     return 0;
 }
 
-int conv_blocks(int w, int h) { return std::max(1, std::min((((w + 15) / 16) * h + kConvWaves - 1) / kConvWaves, 512)); }
+int conv_blocks(int w, int h) { return std::max(1, std::min(((w + 15) / 16) * ((h + kConvWaves - 1) / kConvWaves), 1024)); }
 
 // features + 6 layers; logits in c->logits
 int cnn_forward(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, hipStream_t s)
@@ -489,7 +555,7 @@ int cnn_forward(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, h
         const bool last = i == kLayers - 1;
         if (last)
             hipLaunchKernelGGL(k_conv<1>, dim3(conv_blocks(w, h)), dim3(256), 0, s, c->act[i], w, h,
-                               c->Wf + i * kWPer, c->theta + i * kPPer + kWPer, (const uint16_t *)nullptr,
+                               c->Wf + i * kWPer, c->theta + i * kPPer + kWPer, (const uint16_t *)c->act[0],
                                (void *)c->logits);
         else
             hipLaunchKernelGGL(k_conv<0>, dim3(conv_blocks(w, h)), dim3(256), 0, s, c->act[i], w, h,
@@ -606,7 +672,7 @@ int sgmh_cnn_propose_dev(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W,
     if (cnn_forward(c, l, r, W, H, s))
         return -1;
     const int w = W / 4, h = H / 4, T = ((w + 3) / 4) * ((h + 3) / 4);
-    hipLaunchKernelGGL(k_tile_head, dim3((T + 1) / 2), dim3(64), 0, s, c->logits, w, h, (const int *)nullptr,
+    hipLaunchKernelGGL(k_tile_head, dim3((T + 1) / 2), dim3(64), 0, s, c->logits, w, h, (const uint8_t *)nullptr,
                        thresh, d_blk, d_conf ? d_conf : c->conf, (uint16_t *)nullptr, (float *)nullptr);
     CK(hipGetLastError());
     return 0;
@@ -626,9 +692,8 @@ int sgmh_cnn_train_step_dev(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, con
     if (cnn_forward(c, l, r, W, H, s))
         return -1;
     const int w = W / 4, h = H / 4, T = ((w + 3) / 4) * ((h + 3) / 4);
-    hipLaunchKernelGGL(k_zero_i, dim3((T * kC + 255) / 256), dim3(256), 0, s, c->hist, T * kC);
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, s, c->loss, 1);
-    hipLaunchKernelGGL(k_tile_hist, dim3((W * H + 255) / 256), dim3(256), 0, s, target, W, H, c->hist);
+    hipLaunchKernelGGL(k_pixel_hist, dim3((w * h + 255) / 256), dim3(256), 0, s, target, W, H, c->hist);
     hipLaunchKernelGGL(k_tile_head, dim3((T + 1) / 2), dim3(64), 0, s, c->logits, w, h, c->hist, 2.f, c->blk,
                        c->conf, c->g0, c->loss);
     if (cnn_backward(c, W, H, s))

@@ -33,7 +33,7 @@ def test_gradients_match_finite_differences(exact):
         idx += list(base + rng.integers(0, co.C * co.C * 9, 4)) + [base + co.C * co.C * 9 + int(rng.integers(0, co.C))]
     th64 = theta.astype(np.float64)
     for i in idx:
-        h = 1e-2 * max(1e-2, abs(float(th64[i])))
+        h = 2e-3 * max(1e-2, abs(float(th64[i])))
         tp, tm = th64.copy(), th64.copy()
         tp[i] += h
         tm[i] -= h
@@ -52,7 +52,8 @@ def test_features_closed_form():
     assert np.isclose(f[0, 0, 5], 1.0)
 
 
-def test_head_window_rule():
+def test_head_window_rule(monkeypatch):
+    monkeypatch.setattr(co, "SKIP", 0.0)  # the rule alone, without the feature residual
     theta = np.zeros(co.NPARAM, np.float32)
     # last-layer bias puts all mass on bins 12..15 -> blocks 3 and 4 have the
     # window [48, 80), i.e. b0 = 3
@@ -63,8 +64,30 @@ def test_head_window_rule():
     assert ((blk & 15) == 2).all() or ((blk & 15) == 3).all()
 
 
+def test_feature_residual_orders_bins():
+    """With every weight zero the logits are -SKIP * features: the tile's best
+    bin is the one whose matching cost is lowest."""
+    L, R, _ = sgm_amd.scene_planes(64, 32, 40, 5, planes=(1, 1))
+    theta = np.zeros(co.NPARAM, np.float32)
+    f = co.features(L, R)
+    _, z = co.forward(theta, f)
+    np.testing.assert_allclose(z, -co.SKIP * f, rtol=1e-6)
+
+
 def test_targets_histogram():
     d = np.zeros((16, 32), np.uint8)
     d[:, 16:] = 9
-    q = co.targets(d, 16, 32)
-    assert q.shape == (1, 2, 32) and q[0, 0, 0] == 1.0 and q[0, 1, 2] == 1.0
+    d[0, 0] = 127
+    q = co.targets(d)
+    assert q.shape == (4, 8, 32)
+    assert q[0, 0, 0] == 15 / 16 and q[0, 0, 31] == 1 / 16 and q[1, 0, 0] == 1.0
+    assert q[0, 4, 2] == 1.0 and np.allclose(q.sum(-1), 1.0)
+
+
+def test_tile_probs_is_mean_of_pixel_softmax():
+    z = np.random.default_rng(0).normal(0, 2, (5, 6, 32)).astype(np.float32)
+    P = co.tile_probs(z)
+    assert P.shape == (2, 2, 32)
+    p = co.softmax(z)
+    np.testing.assert_allclose(P[1, 1], p[4:5, 4:6].mean(axis=(0, 1)), rtol=1e-5)
+    np.testing.assert_allclose(P.sum(-1), 1.0, rtol=1e-5)

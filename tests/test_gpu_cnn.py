@@ -100,8 +100,7 @@ def test_proposal_matches_oracle(ctx, H, W):
     net.close()
     np.testing.assert_allclose(conf, conf_w, atol=2e-2)
     # tiles whose decision is not on a knife edge must agree exactly
-    z, _ = co.tile_logits(co.forward(theta, co.features(L, R))[1])
-    p = co.softmax(z)
+    p = co.tile_probs(co.forward(theta, co.features(L, R))[1])
     win = np.sort(np.stack([p[..., 4 * b:4 * b + 8].sum(-1) for b in range(7)], -1), -1)
     clear = (win[..., -1] - win[..., -2] > 0.05) & (np.abs(conf_w - 0.5) > 0.05)
     assert clear.mean() > 0.3
