@@ -429,6 +429,235 @@ __global__ __launch_bounds__(64) void k_tile_head(const float *__restrict__ logi
     }
 }
 
+// ---- fused inference: all six layers and the head in one launch ----
+// A workgroup of 8 waves owns 16x16 low-resolution pixels (4x4 tiles).  Their
+// receptive field through six 3x3 layers is a 28x28 region, staged once in
+// LDS; the layers ping-pong between two LDS copies of that region, so the
+// hidden activations never leave the CU.  Layer i computes the region pixels
+// that layer i+1 reads, as a flat run [29(i+1), 784-29(i+1)) in row-major
+// region order: a 3x3 tap of flat pixel f is f + 28 dy + dx, so 16 consecutive
+// flat pixels are one MFMA operand for every tap (pixels off the valid window
+// wrap across region rows; they are never read by a valid pixel of the next
+// layer).  Pixels outside the image are stored as zeros: the zero padding of
+// every layer.  The last layer computes the 16 inner rows, adds the bias and
+// the residual, and takes the per-pixel softmax in registers (a pixel's 32
+// logits sit in the 4 lanes n, n+16, n+32, n+48).  The head then averages the
+// tile's pixels and picks the window, as k_tile_head does.
+//
+// MFMA v_mfma_f32_16x16x32_bf16: A = weights (row = output channel, 8
+// consecutive input channels of one tap per lane), B = pixels (column = pixel,
+// the same 8 input channels), so the K order inside the instruction is the
+// same for both operands and each lane ends with output channels 4q..4q+3 of
+// pixel n.  A region pixel is 64 bytes (4 chunks of 8 channels); chunk c of
+// flat pixel f sits at chunk slot c ^ ((f >> 1) & 3), so the 16-byte reads of
+// 8 consecutive pixels fall in 8 different bank groups.
+constexpr int kInfOut = 16;
+constexpr int kInfR = kInfOut + 2 * kLayers; // 28
+constexpr int kInfPx = kInfR * kInfR;        // 784
+constexpr int kInfThreads = 512;
+constexpr int kWBytes = kC * 288 * 2;        // one layer's packed weights
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ int inf_slot(int f, int c) { return f * 64 + 16 * (c ^ ((f >> 1) & 3)); }
+
+__global__ __launch_bounds__(kInfThreads) void k_infer(const uint16_t *__restrict__ X, int w, int h,
+                                                       const uint16_t *__restrict__ Wf,
+                                                       const float *__restrict__ theta, float thresh,
+                                                       uint8_t *__restrict__ blk, float *__restrict__ conf)
+{
+    __shared__ __align__(16) uint8_t act[2][kInfPx * 64];
+    __shared__ __align__(16) uint8_t wts[2][kWBytes];
+    __shared__ float bsh[2][kC];
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, n = l & 15, q = l >> 4;
+    const int nbx = (w + kInfOut - 1) / kInfOut;
+    const int bx = (int)blockIdx.x % nbx, by = (int)blockIdx.x / nbx;
+    const int Y0 = by * kInfOut - kLayers, X0 = bx * kInfOut - kLayers;
+    const sgmk::rsrc_t rx = sgmk::make_rsrc(X, (uint32_t)((size_t)w * h * kC * 2));
+    const sgmk::rsrc_t rw = sgmk::make_rsrc(Wf, (uint32_t)(kLayers * kWPer * 2));
+    auto in_image = [&](int f) {
+        const int ry = f / kInfR, Y = Y0 + ry, Xc = X0 + (f - ry * kInfR);
+        return Y >= 0 && Y < h && Xc >= 0 && Xc < w;
+    };
+    auto image_px = [&](int f) {
+        const int ry = f / kInfR;
+        return (Y0 + ry) * w + X0 + (f - ry * kInfR);
+    };
+
+    // features of the region (zeros outside the image), layer 0's weights and bias
+    for (int c = tid; c < kInfPx * 4; c += kInfThreads) {
+        const int f = c >> 2, ch = c & 3;
+        const uint32_t off = in_image(f) ? (uint32_t)((image_px(f) * kC + 8 * ch) * 2) : sgmk::kDrop;
+        *reinterpret_cast<v4u *>(&act[0][inf_slot(f, ch)]) = __builtin_amdgcn_raw_buffer_load_b128(rx, off, 0, 0);
+    }
+    constexpr int kWChunks = kWBytes / 16;                                 // 1152
+    constexpr int kWPerThread = (kWChunks + kInfThreads - 1) / kInfThreads; // 3
+    v4u pw[kWPerThread];
+    float pb = 0.f;
+    auto fetch_layer = [&](int li) {
+#pragma unroll
+        for (int u = 0; u < kWPerThread; u++) {
+            const int c = tid + kInfThreads * u;
+            pw[u] = __builtin_amdgcn_raw_buffer_load_b128(
+                rw, c < kWChunks ? (uint32_t)(li * kWPer * 2 + 16 * c) : sgmk::kDrop, 0, 0);
+        }
+        if (tid < kC)
+            pb = theta[li * kPPer + kWPer + tid];
+    };
+    auto commit_layer = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < kWPerThread; u++) {
+            const int c = tid + kInfThreads * u;
+            if (c < kWChunks)
+                *reinterpret_cast<v4u *>(&wts[buf][16 * c]) = pw[u];
+        }
+        if (tid < kC)
+            bsh[buf][tid] = pb;
+    };
+    fetch_layer(0);
+    commit_layer(0);
+    __syncthreads();
+
+    for (int li = 0; li < kLayers; li++) {
+        const uint8_t *src = act[li & 1];
+        uint8_t *dst = act[(li + 1) & 1];
+        const uint8_t *wl = wts[li & 1];
+        bf16x8 A[9][2];
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+                A[t][nt] = *reinterpret_cast<const bf16x8 *>(wl + ((16 * nt + n) * 288 + 32 * t + 8 * q) * 2);
+        float b[2][4];
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                b[nt][i] = bsh[li & 1][16 * nt + 4 * q + i];
+        if (li + 1 < kLayers)
+            fetch_layer(li + 1); // in flight during this layer's MFMAs
+
+        auto conv16 = [&](int f0, int fmax, f32x4 &acc0, f32x4 &acc1) {
+            const int fr = min(f0 + n, fmax);
+            acc0 = f32x4{0.f, 0.f, 0.f, 0.f};
+            acc1 = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int t = 0; t < 9; t++) {
+                const int g = fr + (t / 3 - 1) * kInfR + (t % 3 - 1);
+                const bf16x8 B = *reinterpret_cast<const bf16x8 *>(src + inf_slot(g, q));
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t][0], B, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t][1], B, acc1, 0, 0, 0);
+            }
+        };
+        if (li + 1 < kLayers) {
+            const int s = 29 * (li + 1), e = kInfPx - 29 * (li + 1), ngr = (e - s + 15) / 16;
+            for (int gi = wv; gi < ngr; gi += kInfThreads / 64) {
+                f32x4 acc[2];
+                conv16(s + 16 * gi, e - 1, acc[0], acc[1]);
+                const int f = s + 16 * gi + n;
+                if (f < e) {
+                    const bool in = in_image(f);
+#pragma unroll
+                    for (int nt = 0; nt < 2; nt++) {
+                        uint32_t pk[2];
+#pragma unroll
+                        for (int i = 0; i < 4; i += 2)
+                            pk[i / 2] = in ? ((uint32_t)to_bf16(fmaxf(acc[nt][i] + b[nt][i], 0.f)) |
+                                              ((uint32_t)to_bf16(fmaxf(acc[nt][i + 1] + b[nt][i + 1], 0.f)) << 16))
+                                           : 0u;
+                        *reinterpret_cast<uint2 *>(dst + inf_slot(f, 2 * nt + (q >> 1)) + 8 * (q & 1)) =
+                            make_uint2(pk[0], pk[1]);
+                    }
+                }
+            }
+        } else {
+            // last layer: inner rows; logits, softmax, probabilities into dst as fp32 [16][16][32]
+            float *prob = reinterpret_cast<float *>(dst);
+            for (int r = wv; r < kInfOut; r += kInfThreads / 64) {
+                f32x4 acc[2];
+                const int f0 = (kLayers + r) * kInfR + kLayers;
+                conv16(f0, kInfPx - 1, acc[0], acc[1]);
+                const int f = f0 + n;
+                const uint32_t base = in_image(f) ? (uint32_t)(image_px(f) * kC * 2) : sgmk::kDrop;
+                float z[2][4];
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++) {
+                    const uint32_t o = base == sgmk::kDrop ? base : base + (uint32_t)((16 * nt + 4 * q) * 2);
+                    const uint2 fm = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rx, o, 0, 0));
+                    const uint16_t fv[4] = {(uint16_t)fm.x, (uint16_t)(fm.x >> 16), (uint16_t)fm.y,
+                                            (uint16_t)(fm.y >> 16)};
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        z[nt][i] = acc[nt][i] + b[nt][i] - kSkip * from_bf16(fv[i]);
+                }
+                float mx = z[0][0];
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        mx = fmaxf(mx, z[nt][i]);
+                mx = fmaxf(mx, __shfl_xor(mx, 16));
+                mx = fmaxf(mx, __shfl_xor(mx, 32));
+                float se = 0.f;
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        z[nt][i] = expf(z[nt][i] - mx);
+                        se += z[nt][i];
+                    }
+                se += __shfl_xor(se, 16);
+                se += __shfl_xor(se, 32);
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++) {
+                    float4 pv;
+                    pv.x = z[nt][0] / se;
+                    pv.y = z[nt][1] / se;
+                    pv.z = z[nt][2] / se;
+                    pv.w = z[nt][3] / se;
+                    *reinterpret_cast<float4 *>(prob + (r * kInfOut + n) * kC + 16 * nt + 4 * q) = pv;
+                }
+            }
+        }
+        if (li + 1 < kLayers)
+            commit_layer((li + 1) & 1);
+        __syncthreads();
+    }
+
+    // head: half-wave (tile, lane = bin) as in k_tile_head
+    const float *prob = reinterpret_cast<const float *>(act[kLayers & 1]);
+    const int tw = (w + 3) / 4, th = (h + 3) / 4;
+    const int tt = tid >> 5, k = tid & 31;
+    const int TY = by * 4 + (tt >> 2), TX = bx * 4 + (tt & 3);
+    float P = 0.f;
+    int cnt = 0;
+    for (int yy = 0; yy < 4; yy++)
+        for (int xx = 0; xx < 4; xx++) {
+            const int y = 4 * TY + yy, x = 4 * TX + xx;
+            if (y < h && x < w) {
+                P += prob[((4 * (tt >> 2) + yy) * kInfOut + 4 * (tt & 3) + xx) * kC + k];
+                cnt++;
+            }
+        }
+    P /= (float)max(cnt, 1);
+    float best = -1.f;
+    int bb = 0;
+    for (int b0 = 0; b0 < kDisp / 16 - 1; b0++) {
+        float m = (k >= 4 * b0 && k < 4 * b0 + 8) ? P : 0.f;
+        for (int s = 16; s; s >>= 1)
+            m += __shfl_xor(m, s, 32);
+        if (m > best) {
+            best = m;
+            bb = b0;
+        }
+    }
+    if (k == 0 && TY < th && TX < tw) {
+        const int t = TY * tw + TX;
+        conf[t] = best;
+        blk[t] = best >= thresh ? (uint8_t)(bb | ((bb + 1) << 4)) : (uint8_t)((kDisp / 16 - 1) << 4);
+    }
+}
+
 // ---- parameters ----
 // forward pack: Wf[co][tap*32 + ci] = W[co][ci][tap]; backward-data pack
 // (conv of dY with flipped taps, channels swapped): Wb[ci][tap*32 + co] = W[co][ci][8 - tap]
@@ -542,8 +771,8 @@ int cnn_pack(sgmh_cnn *c, hipStream_t s)
 
 int conv_blocks(int w, int h) { return std::max(1, std::min(((w + 15) / 16) * ((h + kConvWaves - 1) / kConvWaves), 1024)); }
 
-// features + 6 layers; logits in c->logits
-int cnn_forward(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, hipStream_t s)
+// features (layer 0 input) into c->act[0]; packs the weights if needed
+int cnn_features(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, hipStream_t s)
 {
     const int w = W / 4, h = H / 4, np = w * h;
     if (!c->packed && cnn_pack(c, s))
@@ -551,6 +780,16 @@ int cnn_forward(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, h
     hipLaunchKernelGGL(k_blocksum, dim3((np + 255) / 256), dim3(256), 0, s, l, W, H, 1, c->Ld);
     hipLaunchKernelGGL(k_blocksum, dim3((np * kPhases + 255) / 256), dim3(256), 0, s, r, W, H, kPhases, c->Rd);
     hipLaunchKernelGGL(k_features, dim3((w + kFeatTX - 1) / kFeatTX, h), dim3(256), 0, s, c->Ld, c->Rd, w, h, c->act[0]);
+    CK(hipGetLastError());
+    return 0;
+}
+
+// features + 6 layers, every activation kept for the backward pass; logits in c->logits
+int cnn_forward(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, hipStream_t s)
+{
+    const int w = W / 4, h = H / 4;
+    if (cnn_features(c, l, r, W, H, s))
+        return -1;
     for (int i = 0; i < kLayers; i++) {
         const bool last = i == kLayers - 1;
         if (last)
@@ -669,11 +908,12 @@ int sgmh_cnn_propose_dev(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W,
     if (cnn_check(c, W, H))
         return -1;
     hipStream_t s = (hipStream_t)stream;
-    if (cnn_forward(c, l, r, W, H, s))
+    if (cnn_features(c, l, r, W, H, s))
         return -1;
-    const int w = W / 4, h = H / 4, T = ((w + 3) / 4) * ((h + 3) / 4);
-    hipLaunchKernelGGL(k_tile_head, dim3((T + 1) / 2), dim3(64), 0, s, c->logits, w, h, (const uint8_t *)nullptr,
-                       thresh, d_blk, d_conf ? d_conf : c->conf, (uint16_t *)nullptr, (float *)nullptr);
+    const int w = W / 4, h = H / 4;
+    hipLaunchKernelGGL(k_infer, dim3(((w + kInfOut - 1) / kInfOut) * ((h + kInfOut - 1) / kInfOut)),
+                       dim3(kInfThreads), 0, s, (const uint16_t *)c->act[0], w, h, (const uint16_t *)c->Wf,
+                       (const float *)c->theta, thresh, d_blk, d_conf ? d_conf : c->conf);
     CK(hipGetLastError());
     return 0;
 }

@@ -80,7 +80,7 @@ def train(ctx: Context, net: Cnn, n_pairs: int = 2000, epochs: int = 1, lr: floa
     return losses
 
 
-def heuristic_blk(L: np.ndarray, R: np.ndarray, thresh: float = 0.9):
+def heuristic_blk(L: np.ndarray, R: np.ndarray, thresh: float = 0.9, with_conf: bool = False):
     """The same proposal rule without the network: per tile, the histogram of
     the per-pixel argmin of the network's input features (the low-resolution
     3x3 SAD over 32 bins, best of the 4 sub-block phases) as the
@@ -114,7 +114,21 @@ def heuristic_blk(L: np.ndarray, R: np.ndarray, thresh: float = 0.9):
     p /= p.sum(-1, keepdims=True)
     win = np.stack([p[..., 4 * b:4 * b + 8].sum(-1) for b in range(7)], -1)
     b0, conf = win.argmax(-1), win.max(-1)
-    return np.where(conf >= thresh, b0 | ((b0 + 1) << 4), 7 << 4).astype(np.uint8)
+    blk = np.where(conf >= thresh, b0 | ((b0 + 1) << 4), 7 << 4).astype(np.uint8)
+    return (blk, b0, conf) if with_conf else blk
+
+
+def blk_at_fallback(b0: np.ndarray, conf: np.ndarray, fallback: float) -> np.ndarray:
+    """Block table with the given fraction of tiles at full D: the least
+    confident ones (ties broken by tile order).  Compares proposals at equal
+    cost."""
+    n = conf.size
+    k = int(round(fallback * n))
+    order = np.argsort(conf.ravel(), kind="stable")
+    full = np.zeros(n, bool)
+    full[order[:k]] = True
+    b = b0.ravel()
+    return np.where(full, 7 << 4, b | ((b + 1) << 4)).astype(np.uint8).reshape(conf.shape)
 
 
 def oracle_blk(plain: np.ndarray, cover: float = 1.0) -> np.ndarray:
