@@ -683,15 +683,22 @@ constexpr int kWtaBatch = 2; // pixel batches per wta_batch call
 // A line is cut into 16-step segments, one per tile it crosses; the 4 lines of
 // a wave share every tile (groups of 4 rows or 4 columns never straddle one),
 // so a segment's block range is wave-uniform.  Three segment modes:
-//   NARROW  at most two blocks [b0, b0+1]: lane l holds d = 16*b0 + 2l + j,
-//           j = 0, 1 (the plain layout at DPL = 2 over a 32-disparity window)
+//   NARROW  two blocks [b0, b0+1]: lane l holds d = 16*b0 + 2l + j, j = 0, 1
+//           (the plain layout at DPL = 2 over a 32-disparity window)
 //   FULL    every block: the plain layout at DPL = D/16
-//   MASKED  other ranges: the plain layout, absent disparities pinned to 255
+//   MASKED  other ranges (one block included): the plain layout, absent
+//           disparities pinned to 255
 // The state moves between layouts through a per-wave LDS row (one byte per
 // line and disparity) only where a segment's mode or window differs from the
-// previous one.  Every mode issues the same memory instructions per step (one
-// left load, D/32 16-byte right loads, one store), so the prefetch ring's
-// counters stay exact across mode changes.
+// previous one.
+//
+// Per-step work is what bounds these kernels (a wave issues one instruction
+// per cycle slot, scalar or vector), so every test that only boundary
+// segments need -- steps before or past the line, disparities with x - d < 0,
+// MASKED penalties -- runs in a CHECK variant of the segment body, and the
+// steady-state body has none; addresses are a per-segment base plus constant
+// offsets.  Lane groups past the last line recompute the last line and store
+// the same bytes to the same places.
 constexpr uint32_t kWinAbsent = 255u;
 constexpr uint32_t kWinPenalty = 1000u; // cost of an absent disparity inside a step: L > 255
 
@@ -711,6 +718,8 @@ struct WinSeg {
     uint32_t mask;
 };
 
+// A tile's range is NARROW iff it holds exactly two blocks (the argmin kernel
+// applies the same rule to pick the narrow or the full volume).
 template <int NB>
 __device__ __forceinline__ WinSeg win_seg(uint32_t rng)
 {
@@ -718,206 +727,436 @@ __device__ __forceinline__ WinSeg win_seg(uint32_t rng)
     WinSeg w;
     w.mask = ((2u << b1) - 1u) & ~((1u << b0) - 1u);
     w.b0 = b0;
-    w.mode = b1 - b0 <= 1 ? 0 : (w.mask == (1u << NB) - 1u ? 1 : 2);
+    w.mode = b1 - b0 == 1 ? 0 : (w.mask == (1u << NB) - 1u ? 1 : 2);
     return w;
 }
 
-template <int NB, bool HORIZ>
-__device__ __forceinline__ void path_window_group(const WinArgs &a, const DirTask &t, uint8_t *narrow, int grp)
-{
-    constexpr int PF = 4;
-    constexpr int D = 16 * NB;
-    constexpr int NC = NB / 2; // 16-byte right loads per step (FULL layout)
-    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-    __shared__ __align__(16) uint8_t xs[kLinesPerWave * D]; // layout exchange (one wave per workgroup)
-    const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
-    const int W = a.W, H = a.H;
-    const int dirp = HORIZ ? t.dx : t.dy;
-    const int len = HORIZ ? W : H;
-    const int nt = (len + 15) >> 4;
-    const int line = grp * kLinesPerWave + g;
-    const uint32_t live = line < t.nlines ? 0u : kDrop;
-    const int ln = line < t.nlines ? line : t.nlines - 1;
-    const int fixed_tile = (grp * kLinesPerWave) >> 4;
-    const uint32_t nb8 = (uint32_t)(W * H + kLead) * 8u;
-    const rsrc_t rl = make_rsrc(a.cl, nb8), rr = make_rsrc(a.cr, nb8);
-    const rsrc_t ro = make_rsrc(t.L, a.out_bytes);
-    const rsrc_t rn = make_rsrc(narrow, (uint32_t)(W * H) * 32u);
-    const uint32_t P1 = (uint32_t)t.P1, P2 = kPlainP2, inval = kPlainInvalid;
-    // Steps are counted on a tile-aligned axis q in [0, 16*nt): position
-    // p = q (forward) or 16*nt-1-q (backward); real steps have 0 <= p < len.
-    const int q0 = dirp > 0 ? 0 : 16 * nt - len, qend = dirp > 0 ? len : 16 * nt;
-    auto pos = [&](int q) { return dirp > 0 ? q : 16 * nt - 1 - q; };
-    auto pixel = [&](int p) { return HORIZ ? ln * W + p : p * W + ln; };
-    auto seg_raw = [&](int k) -> uint32_t {
-        const int kk = k < nt ? k : nt - 1;
-        const int along = dirp > 0 ? kk : nt - 1 - kk;
-        const int tx = HORIZ ? along : fixed_tile, ty = HORIZ ? fixed_tile : along;
-        return __builtin_amdgcn_readfirstlane((uint32_t)a.blk[ty * a.tiles_x + tx]);
-    };
-
-    // ring slot k: the left code and the right codes of step q (q % PF == k)
-    uint64_t rcl[PF];
-    v4u rcr[PF][NC];
-    auto fetch = [&](int q, int k, const WinSeg &sg) {
-        const int pix = pixel(pos(q)) + kLead;
-        rcl[k] = ld64(rl, (uint32_t)pix * 8u);
-        // NARROW: codes of d = 16*b0 + 2l + {1, 0} in chunk 0, the other chunks
-        // repeat it; FULL/MASKED: codes of d = NB*l + NB-1-2i, NB-2-2i in chunk
-        // i.  Both modes issue the same memory instructions.
-        const int lo = sg.mode == 0 ? pix - 16 * sg.b0 - 2 * l - 1 : pix - NB * l - (NB - 1);
-        const int st = sg.mode == 0 ? 0 : 2;
+// State of the 4 lines of a wave and its move between segment layouts
+// through the wave's LDS row.  The row is written and read with different
+// access widths, so every access goes through memcpy (byte semantics: no
+// type-based aliasing).  Entering a NARROW window, Lp+P1 of the predecessor's
+// disparities just below / above it become the first step's edge neighbours
+// (its range may reach past the new window).
+template <int NB>
+struct WinState {
+    static constexpr int D = 16 * NB;
+    uint32_t SN[2], SF[NB], mn;
+    uint32_t edge_lo, edge_hi;
+    __device__ __forceinline__ void reset()
+    {
+        SN[0] = SN[1] = 0;
 #pragma unroll
-        for (int i = 0; i < NC; i++)
-            rcr[k][i] = __builtin_amdgcn_raw_buffer_load_b128(rr, (uint32_t)(lo + st * i) * 8u, 0, 0);
-    };
-    auto code = [&](int k, int i, int half) -> uint64_t {
-        const v4u v = rcr[k][i];
-        return half ? ((uint64_t)v[3] << 32) | v[2] : ((uint64_t)v[1] << 32) | v[0];
-    };
-
-    uint32_t SN[2] = {0, 0}, SF[NB], mn = 0;
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-        SF[j] = 0;
-    // Lp+P1 just below / above a NARROW window, for the first step after a
-    // change of window: the predecessor's range may reach past the new window.
-    uint32_t edge_lo = 0xFFFFFFFFu, edge_hi = 0xFFFFFFFFu;
-    // Move the state from the layout of segment `o` to that of segment `c`.
-    // The row is written and read with different access widths, so every
-    // access goes through memcpy (byte semantics: no type-based aliasing).
-    auto put = [](uint8_t *dst, const uint32_t *v, int n) {
+        for (int j = 0; j < NB; j++)
+            SF[j] = 0;
+        mn = 0;
+        edge_lo = edge_hi = 0xFFFFFFFFu;
+    }
+    static __device__ __forceinline__ void put(uint8_t *dst, const uint32_t *v, int n)
+    {
         uint8_t b[NB];
         for (int j = 0; j < n; j++)
             b[j] = (uint8_t)v[j];
         __builtin_memcpy(dst, b, n);
-    };
-    auto get = [](const uint8_t *src, uint32_t *v, int n) {
+    }
+    static __device__ __forceinline__ void get(const uint8_t *src, uint32_t *v, int n)
+    {
         uint8_t b[NB];
         __builtin_memcpy(b, src, n);
         for (int j = 0; j < n; j++)
             v[j] = b[j];
-    };
-    auto exchange = [&](const WinSeg &o, const WinSeg &c) {
-        uint8_t *row = xs + g * D;
+    }
+    __device__ __forceinline__ void exchange(uint8_t *row, int l, uint32_t P1, const WinSeg &o, const WinSeg &c)
+    {
         if (o.mode == 0) {
             uint32_t ff[NB];
 #pragma unroll
             for (int j = 0; j < NB; j++)
                 ff[j] = kWinAbsent;
             put(row + NB * l, ff, NB);
-            const int d = 16 * o.b0 + 2 * l;
-            if (d < D)
-                put(row + d, SN, 2);
+            put(row + 16 * o.b0 + 2 * l, SN, 2);
         } else {
             put(row + NB * l, SF, NB);
         }
         if (c.mode == 0) {
-            const int d = 16 * c.b0 + 2 * l;
-            if (d < D) {
-                get(row + d, SN, 2);
-            } else {
-                SN[0] = kWinAbsent;
-                SN[1] = kWinAbsent;
-            }
+            get(row + 16 * c.b0 + 2 * l, SN, 2);
             const int dl = 16 * c.b0 - 1, dh = 16 * c.b0 + 32;
             edge_lo = dl >= 0 ? (uint32_t)row[dl] + P1 : 0xFFFFFFFFu;
             edge_hi = dh < D ? (uint32_t)row[dh] + P1 : 0xFFFFFFFFu;
         } else {
             get(row + NB * l, SF, NB);
         }
+    }
+};
+
+template <int NB>
+__device__ __forceinline__ bool win_changes(const WinSeg &o, const WinSeg &c)
+{
+    return (c.mode == 0) != (o.mode == 0) || (c.mode == 0 && c.b0 != o.b0);
+}
+
+// Rows: lines grp*4 .. grp*4+3 of a horizontal direction, DIR = +1 (left to
+// right) or -1.  Each lane keeps its window of right codes in registers, as the
+// plain row kernel does: one step along the row shifts it by one disparity
+// through a row_shr:1 / row_shl:1 DPP move, and the end lane takes the one new
+// code (8 bytes, the only right-code load of the step).  The window is a
+// register ring indexed by the step modulo the mode's disparities per lane (2
+// NARROW, NB otherwise; both divide 16).  Each segment starts from a window
+// loaded whole for its mode and window (NB/2 16-byte loads issued PF steps
+// ahead), so every step of every mode issues the same loads in the same
+// order and the prefetch ring's counters are exact.
+template <int NB, int DIR>
+__device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask &t, uint8_t *narrow, int grp)
+{
+    constexpr int PF = 8;
+    constexpr int D = 16 * NB;
+    constexpr int NC = NB / 2;
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    __shared__ __align__(16) uint8_t xs[kLinesPerWave * D]; // layout exchange (one wave per workgroup)
+    __builtin_amdgcn_s_setprio(2); // rows are the frame's longest lines: its critical path
+    const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+    const int W = a.W, H = a.H;
+    const int nt = (W + 15) >> 4;
+    const int ln = min(grp * kLinesPerWave + g, t.nlines - 1);
+    const int tile_y = (grp * kLinesPerWave) >> 4;
+    const uint32_t nb8 = (uint32_t)(W * H + kLead) * 8u;
+    const rsrc_t rl = make_rsrc(a.cl, nb8), rr = make_rsrc(a.cr, nb8);
+    const rsrc_t ro = make_rsrc(t.L, a.out_bytes);
+    const rsrc_t rn = make_rsrc(narrow, (uint32_t)(W * H) * 32u);
+    const uint32_t P1 = (uint32_t)t.P1, P2 = kPlainP2, inval = kPlainInvalid;
+    // steps q in [0, 16*nt) on a tile-aligned axis: x = q (DIR > 0) or 16*nt-1-q
+    const int q0 = DIR > 0 ? 0 : 16 * nt - W, qend = DIR > 0 ? W : 16 * nt;
+    const int rowc = ln * W + kLead, rowp = ln * W; // code / pixel index of (0, ln)
+    auto seg_raw = [&](int k) -> uint32_t {
+        const int kk = k < nt ? k : nt - 1;
+        const int tx = DIR > 0 ? kk : nt - 1 - kk;
+        return __builtin_amdgcn_readfirstlane((uint32_t)a.blk[tile_y * a.tiles_x + tx]);
+    };
+    auto half_code = [](const v4u &v, int half) -> uint64_t {
+        return half ? ((uint64_t)v[3] << 32) | v[2] : ((uint64_t)v[1] << 32) | v[0];
+    };
+    // Step i of a segment (up to 15 + PF for the prefetches into the next one)
+    // lies at x = x0 + DIR*i; offsets are a per-segment base plus a constant,
+    // the base at i = 0 (DIR > 0) or at i = 15 + PF for codes, 15 for stores (DIR < 0).
+    auto cstep = [](int i) { return DIR > 0 ? i : 15 + PF - i; };
+    auto sstep = [](int i) { return DIR > 0 ? i : 15 - i; };
+    // disparity entering at the end lane, and the code offset of a lane's window
+    auto d_enter = [](const WinSeg &sg) {
+        return DIR > 0 ? (sg.mode == 0 ? 16 * sg.b0 : 0) : (sg.mode == 0 ? 16 * sg.b0 + 31 : D - 1);
+    };
+    auto d_lo = [&](const WinSeg &sg) { return sg.mode == 0 ? 16 * sg.b0 + 2 * l + 1 : NB * l + NB - 1; };
+    const uint32_t nbias = l == (DIR > 0 ? 0 : 15) ? 0u : kFar;
+
+    uint64_t rcl[PF], ren[PF];
+    v4u nw[NC];
+    uint64_t win[NB];
+    // per-segment bases (bytes)
+    uint32_t cb, eb_cur, eb_nxt, wb_nxt, sb_n, sb_f;
+    auto bases = [&](int k, const WinSeg &c, const WinSeg &n) {
+        const int x0 = DIR > 0 ? 16 * k : 16 * nt - 1 - 16 * k;
+        const int cbase = rowc + x0 - (DIR > 0 ? 0 : 15 + PF);
+        cb = (uint32_t)cbase * 8u;
+        eb_cur = (uint32_t)(cbase - d_enter(c)) * 8u + nbias;
+        eb_nxt = (uint32_t)(cbase - d_enter(n)) * 8u + nbias;
+        wb_nxt = (uint32_t)(rowc + x0 + 16 * DIR - d_lo(n)) * 8u;
+        const int sbase = rowp + x0 - (DIR > 0 ? 0 : 15);
+        sb_n = (uint32_t)sbase * 32u + (uint32_t)(2 * l);
+        sb_f = (uint32_t)sbase * (uint32_t)D + (uint32_t)(NB * l);
+    };
+    auto reload = [&](const WinSeg &n) {
+        const int st = n.mode == 0 ? 0 : 16;
+#pragma unroll
+        for (int i = 0; i < NC; i++)
+            nw[i] = __builtin_amdgcn_raw_buffer_load_b128(rr, wb_nxt + (uint32_t)(st * i), 0, 0);
+    };
+    // the loads of step i + PF, issued by step i
+    auto prefetch = [&](int i, const WinSeg &n) {
+        const int k = i % PF, cs = cstep(i + PF) * 8;
+        rcl[k] = ld64(rl, cb + (uint32_t)cs);
+        ren[k] = ld64(rr, (i + PF < 16 ? eb_cur : eb_nxt) + (uint32_t)cs);
+        if (i + PF == 16)
+            reload(n);
     };
 
+    WinState<NB> S;
+    S.reset();
+    WinSeg cur = win_seg<NB>(seg_raw(0)), nxt = win_seg<NB>(seg_raw(1)), prev = cur;
+    uint32_t raw_nn = seg_raw(2);
+    // prologue: the loads of segment 0's first PF steps, as the last PF steps
+    // of a segment issue them (a dropped store stands for each step's store)
+    bases(-1, cur, cur);
+#pragma unroll
+    for (int i = 16 - PF; i < 16; i++) {
+        prefetch(i, cur);
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)0, rn, kDrop, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    // one segment: MODE 0 NARROW, 1 FULL / MASKED; CHECK: the boundary tests
+    auto segment = [&](int k, auto mode_tag, auto check_tag) {
+        constexpr int MODE = decltype(mode_tag)::value;
+        constexpr bool CHECK = decltype(check_tag)::value;
+        constexpr int DPL = MODE == 0 ? 2 : NB;
+        const int x0 = DIR > 0 ? 16 * k : 16 * nt - 1 - 16 * k;
+        const int qs = 16 * k;
+        const bool masked = cur.mode == 2;
+        const int dbase = MODE == 0 ? 16 * cur.b0 + 2 * l : NB * l;
+        uint32_t pen[DPL];
+#pragma unroll
+        for (int j = 0; j < DPL; j++)
+            pen[j] = MODE == 1 && masked && !((cur.mask >> ((NB * l + j) >> 4)) & 1u) ? kWinPenalty : 0u;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int k8 = i % PF;
+            if (i == 0) {
+#pragma unroll
+                for (int j = 0; j < DPL; j++)
+                    win[j] = MODE == 0 ? half_code(nw[0], 1 - j) : half_code(nw[(NB - 1 - j) >> 1], (NB - 1 - j) & 1);
+            } else if constexpr (DIR > 0) {
+                const int sl = ((-i) % DPL + DPL) % DPL;
+                win[sl] = dpp64_shr1(ren[k8], win[sl]);
+            } else {
+                const int sl = ((i - 1) % DPL + DPL) % DPL;
+                win[sl] = dpp64_shl1(ren[k8], win[sl]);
+            }
+            uint32_t C[DPL];
+            const uint64_t cL = rcl[k8];
+            const int x = x0 + DIR * i;
+#pragma unroll
+            for (int j = 0; j < DPL; j++) {
+                const uint64_t cr = win[DIR > 0 ? ((j - i) % DPL + DPL) % DPL : (j + i) % DPL];
+                uint32_t c = (uint32_t)__popcll(cL ^ cr);
+                if constexpr (CHECK) {
+                    c = x - dbase >= j ? c : inval;
+                    c = umax(c, pen[j]);
+                }
+                C[j] = c;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            prefetch(i, nxt);
+            const int q = qs + i;
+            const bool real = !CHECK || (q >= q0 && q < qend);
+            if constexpr (MODE == 0) {
+                sgm_step<2, false>(S.SN, C, S.mn, P1, P2, i == 0 ? S.edge_lo : 0xFFFFFFFFu,
+                                   i == 0 ? S.edge_hi : 0xFFFFFFFFu);
+                const uint32_t off = real ? sb_n + (uint32_t)(32 * sstep(i)) : kDrop;
+                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(S.SN[0] | (S.SN[1] << 8)), rn, off, 0, 0);
+                if (CHECK && q < q0) { // before the line's first pixel (backward lines): keep the start state
+                    S.SN[0] = 0;
+                    S.SN[1] = 0;
+                    S.mn = 0;
+                }
+            } else {
+                sgm_step<NB, false>(S.SF, C, S.mn, P1, P2);
+                if constexpr (CHECK) {
+                    if (masked) {
+#pragma unroll
+                        for (int j = 0; j < NB; j++)
+                            S.SF[j] = umin(S.SF[j], kWinAbsent);
+                    }
+                }
+                const uint32_t off = real ? sb_f + (uint32_t)(D * sstep(i)) : kDrop;
+                st_bytes<NB>(ro, off, S.SF);
+                if (CHECK && q < q0) {
+#pragma unroll
+                    for (int j = 0; j < NB; j++)
+                        S.SF[j] = 0;
+                    S.mn = 0;
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        S.edge_lo = 0xFFFFFFFFu;
+        S.edge_hi = 0xFFFFFFFFu;
+    };
+
+#pragma nounroll
+    for (int k = 0; k < nt; k++) {
+        if (k > 0 && win_changes<NB>(prev, cur))
+            S.exchange(xs + g * D, l, P1, prev, cur);
+        bases(k, cur, nxt);
+        // boundary segments: steps outside the line, x within D of the row start
+        const int xmin = DIR > 0 ? 16 * k : 16 * nt - 16 - 16 * k;
+        const bool check = 16 * k < q0 || 16 * k + 16 > qend || xmin < D || cur.mode == 2;
+        if (cur.mode == 0) {
+            if (check)
+                segment(k, std::integral_constant<int, 0>{}, std::true_type{});
+            else
+                segment(k, std::integral_constant<int, 0>{}, std::false_type{});
+        } else {
+            if (check)
+                segment(k, std::integral_constant<int, 1>{}, std::true_type{});
+            else
+                segment(k, std::integral_constant<int, 1>{}, std::false_type{});
+        }
+        prev = cur;
+        cur = nxt;
+        nxt = win_seg<NB>(raw_nn);
+        raw_nn = seg_raw(k + 3);
+    }
+}
+
+// Columns: lines grp*4 .. grp*4+3 of a vertical direction (t.dy = +1 or -1).
+// Each step loads its codes: a NARROW step one 16-byte load per lane, a FULL /
+// MASKED step NB/2.  Inside a segment the count is the segment's own; the
+// last PF steps of a segment prefetch the next segment's first steps with NB/2
+// loads whatever its mode (NARROW reads its codes from the first and repeats
+// the address), so every path into a segment has the same loads in flight and
+// the waits stay exact.  On a column x is fixed, so which disparities have
+// x - d < 0 is fixed too: folded, with the MASKED penalties, into one
+// per-segment floor under the cost.
+template <int NB>
+__device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask &t, uint8_t *narrow, int grp)
+{
+    constexpr int PF = 4;
+    constexpr int D = 16 * NB;
+    constexpr int NC = NB / 2;
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+    __shared__ __align__(16) uint8_t xs[kLinesPerWave * D]; // layout exchange (one wave per workgroup)
+    const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+    const int W = a.W, H = a.H;
+    const int dir = t.dy;
+    const int nt = (H + 15) >> 4;
+    const int ln = min(grp * kLinesPerWave + g, t.nlines - 1); // column x
+    const int tile_x = (grp * kLinesPerWave) >> 4;
+    const uint32_t nb8 = (uint32_t)(W * H + kLead) * 8u;
+    const rsrc_t rl = make_rsrc(a.cl, nb8), rr = make_rsrc(a.cr, nb8);
+    const rsrc_t ro = make_rsrc(t.L, a.out_bytes);
+    const rsrc_t rn = make_rsrc(narrow, (uint32_t)(W * H) * 32u);
+    const uint32_t P1 = (uint32_t)t.P1, P2 = kPlainP2, inval = kPlainInvalid;
+    const int q0 = dir > 0 ? 0 : 16 * nt - H, qend = dir > 0 ? H : 16 * nt;
+    auto seg_raw = [&](int k) -> uint32_t {
+        const int kk = k < nt ? k : nt - 1;
+        const int ty = dir > 0 ? kk : nt - 1 - kk;
+        return __builtin_amdgcn_readfirstlane((uint32_t)a.blk[ty * a.tiles_x + tile_x]);
+    };
+    auto half_code = [](const v4u &v, int half) -> uint64_t {
+        return half ? ((uint64_t)v[3] << 32) | v[2] : ((uint64_t)v[1] << 32) | v[0];
+    };
+    auto d_lo = [&](const WinSeg &sg) { return sg.mode == 0 ? 16 * sg.b0 + 2 * l + 1 : NB * l + NB - 1; };
+    // cursors: the code byte offset of the step being prefetched, the store offsets of the current step
+    const int y0 = dir > 0 ? 0 : 16 * nt - 1;
+    const uint32_t cstep = (uint32_t)(dir * W) * 8u, sstep_n = (uint32_t)(dir * W) * 32u,
+                   sstep_f = (uint32_t)(dir * W) * (uint32_t)D;
+    uint32_t cf = (uint32_t)(y0 * W + ln + kLead) * 8u;
+    uint32_t so_n = (uint32_t)(y0 * W + ln) * 32u + (uint32_t)(2 * l);
+    uint32_t so_f = (uint32_t)(y0 * W + ln) * (uint32_t)D + (uint32_t)(NB * l);
+
+    uint64_t rcl[PF];
+    v4u rcr[PF][NC];
+    auto fetch = [&](int k, const WinSeg &sg, auto n_tag) {
+        constexpr int N = decltype(n_tag)::value;
+        rcl[k] = ld64(rl, cf);
+        const uint32_t lo = cf - (uint32_t)d_lo(sg) * 8u;
+        const int st = sg.mode == 0 ? 0 : 16;
+#pragma unroll
+        for (int i = 0; i < N; i++)
+            rcr[k][i] = __builtin_amdgcn_raw_buffer_load_b128(rr, lo + (uint32_t)(st * i), 0, 0);
+        cf += cstep;
+    };
+    using kOne = std::integral_constant<int, 1>;
+    using kAll = std::integral_constant<int, NC>;
+
+    WinState<NB> S;
+    S.reset();
     WinSeg cur = win_seg<NB>(seg_raw(0)), nxt = win_seg<NB>(seg_raw(1)), prev = cur;
     uint32_t raw_nn = seg_raw(2);
 #pragma unroll
     for (int k = 0; k < PF; k++) {
-        fetch(k, k, cur);
+        fetch(k, cur, kAll{});
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)0, rn, kDrop, 0, 0);
         __builtin_amdgcn_sched_barrier(0);
     }
 
-    for (int k = 0; k < nt; k++) {
-        if (k > 0 && (cur.mode == 0) != (prev.mode == 0))
-            exchange(prev, cur);
-        else if (k > 0 && cur.mode == 0 && cur.b0 != prev.b0)
-            exchange(prev, cur);
+    auto segment = [&](int k, auto mode_tag, auto check_tag) {
+        constexpr int MODE = decltype(mode_tag)::value;
+        constexpr bool CHECK = decltype(check_tag)::value;
+        constexpr int DPL = MODE == 0 ? 2 : NB;
         const int qs = 16 * k;
+        const bool masked = cur.mode == 2;
+        const int dbase = MODE == 0 ? 16 * cur.b0 + 2 * l : NB * l;
+        uint32_t floor_[DPL];
+#pragma unroll
+        for (int j = 0; j < DPL; j++) {
+            const uint32_t pen =
+                MODE == 1 && masked && !((cur.mask >> ((NB * l + j) >> 4)) & 1u) ? kWinPenalty : 0u;
+            floor_[j] = umax(ln - dbase >= j ? 0u : inval, pen);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int rk = i % PF;
+            uint32_t C[DPL];
+            const uint64_t cL = rcl[rk];
+#pragma unroll
+            for (int j = 0; j < DPL; j++) {
+                const uint64_t cr = MODE == 0 ? half_code(rcr[rk][0], 1 - j)
+                                              : half_code(rcr[rk][(NB - 1 - j) >> 1], (NB - 1 - j) & 1);
+                uint32_t c = (uint32_t)__popcll(cL ^ cr);
+                if constexpr (CHECK)
+                    c = umax(c, floor_[j]);
+                C[j] = c;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            if (i + PF < 16) {
+                if constexpr (MODE == 0)
+                    fetch(rk, cur, kOne{});
+                else
+                    fetch(rk, cur, kAll{});
+            } else {
+                fetch(rk, nxt, kAll{});
+            }
+            const int q = qs + i;
+            const bool real = !CHECK || (q >= q0 && q < qend);
+            if constexpr (MODE == 0) {
+                sgm_step<2, false>(S.SN, C, S.mn, P1, P2, i == 0 ? S.edge_lo : 0xFFFFFFFFu,
+                                   i == 0 ? S.edge_hi : 0xFFFFFFFFu);
+                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(S.SN[0] | (S.SN[1] << 8)), rn, real ? so_n : kDrop, 0, 0);
+                if (CHECK && q < q0) {
+                    S.SN[0] = 0;
+                    S.SN[1] = 0;
+                    S.mn = 0;
+                }
+            } else {
+                sgm_step<NB, false>(S.SF, C, S.mn, P1, P2);
+                if constexpr (CHECK) {
+                    if (masked) {
+#pragma unroll
+                        for (int j = 0; j < NB; j++)
+                            S.SF[j] = umin(S.SF[j], kWinAbsent);
+                    }
+                }
+                st_bytes<NB>(ro, real ? so_f : kDrop, S.SF);
+                if (CHECK && q < q0) {
+#pragma unroll
+                    for (int j = 0; j < NB; j++)
+                        S.SF[j] = 0;
+                    S.mn = 0;
+                }
+            }
+            so_n += sstep_n;
+            so_f += sstep_f;
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        S.edge_lo = 0xFFFFFFFFu;
+        S.edge_hi = 0xFFFFFFFFu;
+    };
+
+    // x >= D - 1 on every line of the wave: no disparity is ever invalid
+    const bool all_valid = grp * kLinesPerWave >= D - 1;
+#pragma nounroll
+    for (int k = 0; k < nt; k++) {
+        if (k > 0 && win_changes<NB>(prev, cur))
+            S.exchange(xs + g * D, l, P1, prev, cur);
+        const bool check = 16 * k < q0 || 16 * k + 16 > qend || !all_valid || cur.mode == 2;
         if (cur.mode == 0) {
-            // NARROW: the second block may be absent (range of one block)
-            const bool one = !((cur.mask >> (cur.b0 + 1)) & 1u);
-            const uint32_t pen = one && l >= 8 ? kWinPenalty : 0u;
-            const int dbase = 16 * cur.b0 + 2 * l;
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int q = qs + i, rk = i % PF;
-                const int p = pos(q);
-                const int xq = HORIZ ? p : ln;
-                uint32_t C[2];
-                const uint64_t cL = rcl[rk];
-#pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    const uint32_t h = (uint32_t)__popcll(cL ^ code(rk, 0, 1 - j));
-                    const uint32_t c = xq - dbase >= j ? h : inval;
-                    C[j] = c > pen ? c : pen;
-                }
-                __builtin_amdgcn_sched_barrier(0);
-                fetch(q + PF, rk, i + PF < 16 ? cur : nxt);
-                sgm_step<2, false>(SN, C, mn, P1, P2, edge_lo, edge_hi);
-                edge_lo = 0xFFFFFFFFu;
-                edge_hi = 0xFFFFFFFFu;
-                if (one) {
-                    SN[0] = umin(SN[0], kWinAbsent);
-                    SN[1] = umin(SN[1], kWinAbsent);
-                }
-                const bool real = q >= q0 && q < qend;
-                const uint32_t off = real ? ((uint32_t)pixel(p) * 32u + (uint32_t)(2 * l)) | live : kDrop;
-                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(SN[0] | (SN[1] << 8)), rn, off, 0, 0);
-                if (q < q0) { // before the line's first pixel (backward lines): keep the start state
-                    SN[0] = 0;
-                    SN[1] = 0;
-                    mn = 0;
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            if (check)
+                segment(k, std::integral_constant<int, 0>{}, std::true_type{});
+            else
+                segment(k, std::integral_constant<int, 0>{}, std::false_type{});
         } else {
-            const bool masked = cur.mode == 2;
-            uint32_t pen[NB];
-#pragma unroll
-            for (int j = 0; j < NB; j++)
-                pen[j] = masked && !((cur.mask >> ((NB * l + j) >> 4)) & 1u) ? kWinPenalty : 0u;
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int q = qs + i, rk = i % PF;
-                const int p = pos(q);
-                const int xq = HORIZ ? p : ln;
-                uint32_t C[NB];
-                const uint64_t cL = rcl[rk];
-#pragma unroll
-                for (int j = 0; j < NB; j++) {
-                    const int c16 = (NB - 1 - j) >> 1, half = (NB - 1 - j) & 1; // chunk, low/high
-                    const uint32_t h = (uint32_t)__popcll(cL ^ code(rk, c16, half));
-                    const uint32_t c = xq - NB * l >= j ? h : inval;
-                    C[j] = c > pen[j] ? c : pen[j];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-                fetch(q + PF, rk, i + PF < 16 ? cur : nxt);
-                sgm_step<NB, false>(SF, C, mn, P1, P2);
-                if (masked) {
-#pragma unroll
-                    for (int j = 0; j < NB; j++)
-                        SF[j] = umin(SF[j], kWinAbsent);
-                }
-                const bool real = q >= q0 && q < qend;
-                const uint32_t off = real ? ((uint32_t)pixel(p) * (uint32_t)D + (uint32_t)(NB * l)) | live : kDrop;
-                st_bytes<NB>(ro, off, SF);
-                if (q < q0) {
-#pragma unroll
-                    for (int j = 0; j < NB; j++)
-                        SF[j] = 0;
-                    mn = 0;
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            if (check)
+                segment(k, std::integral_constant<int, 1>{}, std::true_type{});
+            else
+                segment(k, std::integral_constant<int, 1>{}, std::false_type{});
         }
         prev = cur;
         cur = nxt;
@@ -928,7 +1167,7 @@ __device__ __forceinline__ void path_window_group(const WinArgs &a, const DirTas
 
 // Sum over the directions and argmin over the pixel's active range, one tile
 // (16x16 pixels, uniform range) at a time per wave.  One lane per active
-// 16-disparity block: a range of at most two blocks puts 2 lanes on a pixel
+// 16-disparity block: a range of exactly two blocks puts 2 lanes on a pixel
 // (32 pixels per pass) reading the dense narrow volumes, a wider one NB lanes
 // (64/NB pixels per pass) reading the full ones; lanes of inactive blocks load
 // nothing and bid the largest key.
@@ -967,7 +1206,7 @@ __device__ __forceinline__ void wta_window_tile(const uint8_t *__restrict__ L, s
     const uint32_t rng = __builtin_amdgcn_readfirstlane((uint32_t)blk[t]);
     const int b1 = min((int)(rng >> 4), NB - 1), b0 = min((int)(rng & 15u), b1);
     const int ty = t / tiles_x, tx = t - ty * tiles_x;
-    if (b1 - b0 <= 1) {
+    if (b1 - b0 == 1) {
         const int b = b0 + (lane & 1);
         const bool act = b <= b1;
 #pragma unroll 2

@@ -123,10 +123,14 @@ __global__ __launch_bounds__(64) void k_path_window(WinArgs a)
         k++;
     const DirTask t = a.dir[k];
     const int grp = (int)blockIdx.x - t.first_block;
-    if (t.dy == 0)
-        path_window_group<NB, true>(a, t, a.Ln[k], grp);
-    else
-        path_window_group<NB, false>(a, t, a.Ln[k], grp);
+    if (t.dy == 0) {
+        if (t.dx > 0)
+            path_window_rows<NB, 1>(a, t, a.Ln[k], grp);
+        else
+            path_window_rows<NB, -1>(a, t, a.Ln[k], grp);
+    } else {
+        path_window_cols<NB>(a, t, a.Ln[k], grp);
+    }
 }
 
 template <int NB>

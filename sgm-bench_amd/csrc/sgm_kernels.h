@@ -19,6 +19,7 @@ constexpr int kRowLanes = 16;    // lanes per scanline (one DPP row)
 constexpr int kLinesPerWave = 4; // scanlines per wave64
 
 __device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
 
 // DPP move with every lane valid (bound_ctrl irrelevant for these patterns).
 template <int CTRL>
