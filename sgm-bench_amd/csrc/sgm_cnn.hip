@@ -51,82 +51,95 @@ __device__ __forceinline__ uint16_t to_bf16(float f)
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // ---- features ----
-// 4x4 block sums of an image at phase `ph`: block x covers full-resolution
-// columns 4x - ph .. 4x - ph + 3 (clamped at the image edges).  Phase 0 is
-// the plain 4x downsampling; phases 1-3 are the right image's sub-block
-// shifts, so that bin k can match full-resolution disparities 4k .. 4k+3.
-__global__ __launch_bounds__(256) void k_blocksum(const uint8_t *__restrict__ img, int W, int H, int phases,
-                                                  int32_t *__restrict__ out)
+// f_k(y, x) = min over the 4 phases ph of sum over the clamped 3x3 window of
+// |Ld - Rd_ph(x' - k)| (4080 where x' - k < 0), divided by 9*4080, bf16.  Ld is
+// the 4x4 block sum of the left image; Rd_ph that of the right image at phase
+// ph: block x covers full-resolution columns 4x - ph .. 4x - ph + 3, clamped at
+// 0, so bin k matches full-resolution disparities 4k + ph.
+//
+// A workgroup makes 8 rows x 32 columns of pixels, all 32 bins.  It first
+// forms the block sums it needs (rows y0-1 .. y0+8, clamped) into LDS straight
+// from the images: Ld from one aligned word per image row, all four Rd_ph from
+// two (a byte-aligned window per phase, v_alignbyte) -- v_dot4 with ones sums
+// the bytes.  Thread (k, column group) then takes columns one at a time: per
+// row and phase the three horizontal differences of the window (v_sad), then
+// the vertical sum of three rows, so each output costs a few operations per
+// phase instead of nine.  The 32 threads of a pixel write its 64 bytes
+// contiguously.
+constexpr int kFeatTX = 32, kFeatTY = 8;
+constexpr int kPhases = 4;
+__device__ __forceinline__ uint32_t sum4(uint32_t v, uint32_t acc) { return __builtin_amdgcn_udot4(v, 0x01010101u, acc, false); }
+
+__global__ __launch_bounds__(256) void k_features(const uint8_t *__restrict__ Lim, const uint8_t *__restrict__ Rim,
+                                                  int W, int H, uint16_t *__restrict__ X)
 {
     const int w = W / 4, h = H / 4;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= w * h * phases)
-        return;
-    const int ph = i / (w * h), r = i - ph * (w * h);
-    const int y = r / w, x = r - y * w;
-    int s = 0;
-    for (int dy = 0; dy < 4; dy++)
-        for (int dx = 0; dx < 4; dx++)
-            s += img[(size_t)(4 * y + dy) * W + max(4 * x + dx - ph, 0)];
-    out[i] = s;
-}
-
-// f_k = min over the 4 phases ph of sum_{clamped 3x3} |Ld - Rd_ph(x'-k)|
-// (4080 where x'-k < 0), divided by 9*4080, bf16: bin k covers full-resolution
-// disparities 4k + ph.  A workgroup makes 32 pixels of one row, all 32 bins:
-// the three rows of Ld it needs (x0-1 .. x0+32) and of each Rd_ph (x0-32 ..
-// x0+32) are staged in LDS; thread (k, s) makes bin k of pixels x0 + 4s ..
-// x0 + 4s + 3, so the 32 threads of one pixel write its 64 bytes contiguously.
-constexpr int kFeatTX = 32;
-constexpr int kPhases = 4;
-__global__ __launch_bounds__(256) void k_features(const int32_t *__restrict__ Ld, const int32_t *__restrict__ Rd,
-                                                  int w, int h, uint16_t *__restrict__ X)
-{
-    __shared__ int32_t sl[3][kFeatTX + 2], sr[kPhases][3][kFeatTX + 33];
-    const int x0 = blockIdx.x * kFeatTX, y = blockIdx.y;
-    for (int i = threadIdx.x; i < 3 * (kFeatTX + 2); i += 256) {
-        const int r = i / (kFeatTX + 2), c = i - r * (kFeatTX + 2);
-        const int yy = min(max(y + r - 1, 0), h - 1), xx = min(max(x0 - 1 + c, 0), w - 1);
-        sl[r][c] = Ld[yy * w + xx];
+    constexpr int LC = kFeatTX + 2, RC = kFeatTX + 33, NR = kFeatTY + 2;
+    __shared__ int sl[NR][LC];              // Ld, columns x0-1 .. x0+32
+    __shared__ int sr[kPhases][NR][RC];     // Rd_ph, columns x0-32 .. x0+32
+    const int x0 = blockIdx.x * kFeatTX, y0 = blockIdx.y * kFeatTY;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NR * LC; i += 256) {
+        const int r = i / LC, c = i - r * LC;
+        const int Y = min(max(y0 - 1 + r, 0), h - 1), Xc = min(max(x0 - 1 + c, 0), w - 1);
+        uint32_t acc = 0;
+#pragma unroll
+        for (int dy = 0; dy < 4; dy++)
+            acc = sum4(*reinterpret_cast<const uint32_t *>(Lim + (size_t)(4 * Y + dy) * W + 4 * Xc), acc);
+        sl[r][c] = (int)acc;
     }
-    for (int i = threadIdx.x; i < kPhases * 3 * (kFeatTX + 33); i += 256) {
-        const int ph = i / (3 * (kFeatTX + 33)), rc = i - ph * (3 * (kFeatTX + 33));
-        const int r = rc / (kFeatTX + 33), c = rc - r * (kFeatTX + 33);
-        const int yy = min(max(y + r - 1, 0), h - 1), xx = min(max(x0 - 32 + c, 0), w - 1);
-        sr[ph][r][c] = Rd[(size_t)ph * w * h + yy * w + xx];
-    }
-    __syncthreads();
-    const int k = threadIdx.x & 31, sub = threadIdx.x >> 5;
-    const int xb = x0 + 4 * sub;
-    // the six clamped columns xb-1 .. xb+4 that the thread's 4 pixels' 3x3
-    // windows touch; each column's difference is formed once per row and phase
-    int cx[6];
+    for (int i = tid; i < NR * RC; i += 256) {
+        const int r = i / RC, c = i - r * RC;
+        const int Y = min(max(y0 - 1 + r, 0), h - 1), Xc = min(max(x0 - 32 + c, 0), w - 1);
+        uint32_t acc[kPhases] = {0, 0, 0, 0};
 #pragma unroll
-    for (int j = 0; j < 6; j++)
-        cx[j] = min(max(xb - 1 + j, 0), w - 1);
-    int best[4] = {0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF, 0x7FFFFFFF};
+        for (int dy = 0; dy < 4; dy++) {
+            const uint8_t *row = Rim + (size_t)(4 * Y + dy) * W;
+            const uint32_t hi = *reinterpret_cast<const uint32_t *>(row + 4 * Xc);
+            // columns 4Xc-4 .. 4Xc-1; left of the image they repeat column 0
+            const uint32_t lo = Xc > 0 ? *reinterpret_cast<const uint32_t *>(row + 4 * Xc - 4) : (uint32_t)row[0] * 0x01010101u;
+            acc[0] = sum4(hi, acc[0]);
 #pragma unroll
-    for (int ph = 0; ph < kPhases; ph++) {
-        int acc[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-            int dcol[6];
-#pragma unroll
-            for (int j = 0; j < 6; j++)
-                dcol[j] = cx[j] - k >= 0 ? abs(sl[r][cx[j] - x0 + 1] - sr[ph][r][cx[j] - k - x0 + 32]) : 4080;
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                acc[u] += dcol[u] + dcol[u + 1] + dcol[u + 2];
+            for (int ph = 1; ph < kPhases; ph++)
+                acc[ph] = sum4(__builtin_amdgcn_alignbyte(hi, lo, 4 - ph), acc[ph]);
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-            best[u] = min(best[u], acc[u]);
+        for (int ph = 0; ph < kPhases; ph++)
+            sr[ph][r][c] = (int)acc[ph];
     }
+    __syncthreads();
+    const int k = tid & 31, cg = tid >> 5;
+    for (int xl = cg; xl < kFeatTX; xl += 8) {
+        const int x = x0 + xl;
+        if (x >= w)
+            break;
+        int cx[3];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int x = xb + u;
-        if (x < w)
-            X[((size_t)y * w + x) * kC + k] = to_bf16((float)best[u] / (float)(9 * 4080));
+        for (int j = 0; j < 3; j++)
+            cx[j] = min(max(x - 1 + j, 0), w - 1);
+        int best[kFeatTY];
+#pragma unroll
+        for (int y = 0; y < kFeatTY; y++)
+            best[y] = 0x7FFFFFFF;
+#pragma unroll
+        for (int ph = 0; ph < kPhases; ph++) {
+            int h3[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                int acc = 0;
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    acc += cx[j] - k >= 0 ? abs(sl[r][cx[j] - x0 + 1] - sr[ph][r][cx[j] - k - x0 + 32]) : 4080;
+                h3[r] = acc;
+            }
+#pragma unroll
+            for (int y = 0; y < kFeatTY; y++)
+                best[y] = min(best[y], h3[y] + h3[y + 1] + h3[y + 2]);
+        }
+#pragma unroll
+        for (int y = 0; y < kFeatTY; y++)
+            if (y0 + y < h)
+                X[((size_t)(y0 + y) * w + x) * kC + k] = to_bf16((float)best[y] / (float)(9 * 4080));
     }
 }
 
@@ -706,7 +719,6 @@ struct sgmh_cnn {
     bool packed = false;
     // activations for the current frame size
     int W = 0, H = 0;
-    int32_t *Ld = nullptr, *Rd = nullptr;
     uint16_t *act[kLayers] = {};   // act[i] = input of layer i (bf16 NHWC)
     float *logits = nullptr;
     uint16_t *g0 = nullptr, *g1 = nullptr; // gradient ping-pong (bf16)
@@ -723,7 +735,7 @@ int cnn_resize(sgmh_cnn *c, int W, int H)
     if (c->W == W && c->H == H)
         return 0;
     CK(hipDeviceSynchronize());
-    void **bufs[] = {(void **)&c->Ld, (void **)&c->Rd, (void **)&c->logits, (void **)&c->g0, (void **)&c->g1,
+    void **bufs[] = {(void **)&c->logits, (void **)&c->g0, (void **)&c->g1,
                      (void **)&c->hist, (void **)&c->loss, (void **)&c->blk, (void **)&c->conf};
     for (void **b : bufs)
         if (*b) {
@@ -736,8 +748,6 @@ int cnn_resize(sgmh_cnn *c, int W, int H)
             a = nullptr;
         }
     const size_t w = W / 4, h = H / 4, np = w * h, T = ((w + 3) / 4) * ((h + 3) / 4);
-    CK(hipMalloc(&c->Ld, np * 4));
-    CK(hipMalloc(&c->Rd, np * 4 * kPhases));
     for (auto &a : c->act)
         CK(hipMalloc(&a, np * kC * 2));
     CK(hipMalloc(&c->logits, np * kC * 4));
@@ -774,12 +784,12 @@ int conv_blocks(int w, int h) { return std::max(1, std::min(((w + 15) / 16) * ((
 // features (layer 0 input) into c->act[0]; packs the weights if needed
 int cnn_features(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, hipStream_t s)
 {
-    const int w = W / 4, h = H / 4, np = w * h;
+    const int w = W / 4, h = H / 4;
+    REQUIRE((((uintptr_t)l | (uintptr_t)r) & 3) == 0, "sgmh_cnn: images must be 4-byte aligned");
     if (!c->packed && cnn_pack(c, s))
         return -1;
-    hipLaunchKernelGGL(k_blocksum, dim3((np + 255) / 256), dim3(256), 0, s, l, W, H, 1, c->Ld);
-    hipLaunchKernelGGL(k_blocksum, dim3((np * kPhases + 255) / 256), dim3(256), 0, s, r, W, H, kPhases, c->Rd);
-    hipLaunchKernelGGL(k_features, dim3((w + kFeatTX - 1) / kFeatTX, h), dim3(256), 0, s, c->Ld, c->Rd, w, h, c->act[0]);
+    hipLaunchKernelGGL(k_features, dim3((w + kFeatTX - 1) / kFeatTX, (h + kFeatTY - 1) / kFeatTY), dim3(256), 0, s,
+                       l, r, W, H, c->act[0]);
     CK(hipGetLastError());
     return 0;
 }
@@ -868,7 +878,7 @@ void sgmh_cnn_destroy(sgmh_cnn *c)
         return;
     (void)hipSetDevice(c->ctx->device);
     (void)hipDeviceSynchronize();
-    void *bufs[] = {c->theta, c->grad, c->m, c->v, c->Wf, c->Wb, c->Ld, c->Rd, c->logits, c->g0, c->g1,
+    void *bufs[] = {c->theta, c->grad, c->m, c->v, c->Wf, c->Wb, c->logits, c->g0, c->g1,
                     c->hist, c->loss, c->blk, c->conf};
     for (void *b : bufs)
         if (b)
