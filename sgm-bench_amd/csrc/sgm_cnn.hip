@@ -443,9 +443,9 @@ __global__ __launch_bounds__(64) void k_tile_head(const float *__restrict__ logi
 }
 
 // ---- fused inference: all six layers and the head in one launch ----
-// A workgroup of 8 waves owns 16x16 low-resolution pixels (4x4 tiles).  Their
-// receptive field through six 3x3 layers is a 28x28 region, staged once in
-// LDS; the layers ping-pong between two LDS copies of that region, so the
+// A workgroup of 16 waves owns 16x16 low-resolution pixels (4x4 tiles).
+// Their receptive field through six 3x3 layers is a 28x28 region, staged once
+// in LDS; the layers ping-pong between two LDS copies of that region, so the
 // hidden activations never leave the CU.  Layer i computes the region pixels
 // that layer i+1 reads, as a flat run [29(i+1), 784-29(i+1)) in row-major
 // region order: a 3x3 tap of flat pixel f is f + 28 dy + dx, so 16 consecutive
@@ -461,28 +461,35 @@ __global__ __launch_bounds__(64) void k_tile_head(const float *__restrict__ logi
 // consecutive input channels of one tap per lane), B = pixels (column = pixel,
 // the same 8 input channels), so the K order inside the instruction is the
 // same for both operands and each lane ends with output channels 4q..4q+3 of
-// pixel n.  A region pixel is 64 bytes (4 chunks of 8 channels); chunk c of
-// flat pixel f sits at chunk slot c ^ ((f >> 1) & 3), so the 16-byte reads of
-// 8 consecutive pixels fall in 8 different bank groups.
+// pixel n.  A region pixel takes 80 bytes (64 of channels and 16 of padding):
+// 16-byte reads of 16 consecutive pixels then fall in 16 different bank
+// groups, and every tap of a lane is its base address plus a constant.
 constexpr int kInfOut = 16;
 constexpr int kInfR = kInfOut + 2 * kLayers; // 28
 constexpr int kInfPx = kInfR * kInfR;        // 784
-constexpr int kInfThreads = 512;
+constexpr int kInfStride = 80;               // bytes per region pixel
+constexpr int kInfThreads = 1024;
 constexpr int kWBytes = kC * 288 * 2;        // one layer's packed weights
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ int inf_slot(int f, int c) { return f * 64 + 16 * (c ^ ((f >> 1) & 3)); }
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2)); // v_cvt_pk_bf16_f32, RNE
+}
 
 __global__ __launch_bounds__(kInfThreads) void k_infer(const uint16_t *__restrict__ X, int w, int h,
                                                        const uint16_t *__restrict__ Wf,
                                                        const float *__restrict__ theta, float thresh,
                                                        uint8_t *__restrict__ blk, float *__restrict__ conf)
 {
-    __shared__ __align__(16) uint8_t act[2][kInfPx * 64];
-    __shared__ __align__(16) uint8_t wts[2][kWBytes];
+    __shared__ __align__(16) uint8_t act[2][kInfPx * kInfStride];
+    __shared__ __align__(16) uint8_t wts[kWBytes];
     __shared__ float bsh[2][kC];
     typedef uint32_t v4u __attribute__((ext_vector_type(4)));
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, n = l & 15, q = l >> 4;
+    constexpr int kWaves = kInfThreads / 64;
     const int nbx = (w + kInfOut - 1) / kInfOut;
     const int bx = (int)blockIdx.x % nbx, by = (int)blockIdx.x / nbx;
     const int Y0 = by * kInfOut - kLayers, X0 = bx * kInfOut - kLayers;
@@ -497,14 +504,8 @@ __global__ __launch_bounds__(kInfThreads) void k_infer(const uint16_t *__restric
         return (Y0 + ry) * w + X0 + (f - ry * kInfR);
     };
 
-    // features of the region (zeros outside the image), layer 0's weights and bias
-    for (int c = tid; c < kInfPx * 4; c += kInfThreads) {
-        const int f = c >> 2, ch = c & 3;
-        const uint32_t off = in_image(f) ? (uint32_t)((image_px(f) * kC + 8 * ch) * 2) : sgmk::kDrop;
-        *reinterpret_cast<v4u *>(&act[0][inf_slot(f, ch)]) = __builtin_amdgcn_raw_buffer_load_b128(rx, off, 0, 0);
-    }
     constexpr int kWChunks = kWBytes / 16;                                 // 1152
-    constexpr int kWPerThread = (kWChunks + kInfThreads - 1) / kInfThreads; // 3
+    constexpr int kWPerThread = (kWChunks + kInfThreads - 1) / kInfThreads; // 2
     v4u pw[kWPerThread];
     float pb = 0.f;
     auto fetch_layer = [&](int li) {
@@ -517,76 +518,77 @@ __global__ __launch_bounds__(kInfThreads) void k_infer(const uint16_t *__restric
         if (tid < kC)
             pb = theta[li * kPPer + kWPer + tid];
     };
-    auto commit_layer = [&](int buf) {
+    auto commit_layer = [&](int li) {
 #pragma unroll
         for (int u = 0; u < kWPerThread; u++) {
             const int c = tid + kInfThreads * u;
             if (c < kWChunks)
-                *reinterpret_cast<v4u *>(&wts[buf][16 * c]) = pw[u];
+                *reinterpret_cast<v4u *>(&wts[16 * c]) = pw[u];
         }
         if (tid < kC)
-            bsh[buf][tid] = pb;
+            bsh[li & 1][tid] = pb;
     };
+    // features of the region (zeros outside the image), layer 0's weights and bias
     fetch_layer(0);
+    for (int c = tid; c < kInfPx * 4; c += kInfThreads) {
+        const int f = c >> 2, ch = c & 3;
+        const uint32_t off = in_image(f) ? (uint32_t)((image_px(f) * kC + 8 * ch) * 2) : sgmk::kDrop;
+        *reinterpret_cast<v4u *>(&act[0][f * kInfStride + 16 * ch]) =
+            __builtin_amdgcn_raw_buffer_load_b128(rx, off, 0, 0);
+    }
     commit_layer(0);
     __syncthreads();
 
     for (int li = 0; li < kLayers; li++) {
         const uint8_t *src = act[li & 1];
         uint8_t *dst = act[(li + 1) & 1];
-        const uint8_t *wl = wts[li & 1];
         bf16x8 A[9][2];
 #pragma unroll
         for (int t = 0; t < 9; t++)
 #pragma unroll
             for (int nt = 0; nt < 2; nt++)
-                A[t][nt] = *reinterpret_cast<const bf16x8 *>(wl + ((16 * nt + n) * 288 + 32 * t + 8 * q) * 2);
-        float b[2][4];
-#pragma unroll
-        for (int nt = 0; nt < 2; nt++)
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-                b[nt][i] = bsh[li & 1][16 * nt + 4 * q + i];
+                A[t][nt] = *reinterpret_cast<const bf16x8 *>(wts + ((16 * nt + n) * 288 + 32 * t + 8 * q) * 2);
+        // this lane's 8 biases, read from LDS where they are used (registers are the limit)
+        auto bias4 = [&](int nt) { return *reinterpret_cast<const float4 *>(&bsh[li & 1][16 * nt + 4 * q]); };
         if (li + 1 < kLayers)
             fetch_layer(li + 1); // in flight during this layer's MFMAs
+        __syncthreads();         // every wave holds this layer's weights: the buffer is free
 
+        // 16 pixels from flat pixel f0 (reads clamped at fmax); taps are constants from one base
         auto conv16 = [&](int f0, int fmax, f32x4 &acc0, f32x4 &acc1) {
             const int fr = min(f0 + n, fmax);
+            const uint8_t *base = src + (fr - kInfR - 1) * kInfStride + 16 * q;
             acc0 = f32x4{0.f, 0.f, 0.f, 0.f};
             acc1 = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int t = 0; t < 9; t++) {
-                const int g = fr + (t / 3 - 1) * kInfR + (t % 3 - 1);
-                const bf16x8 B = *reinterpret_cast<const bf16x8 *>(src + inf_slot(g, q));
+                const bf16x8 B = *reinterpret_cast<const bf16x8 *>(base + ((t / 3) * kInfR + (t % 3)) * kInfStride);
                 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t][0], B, acc0, 0, 0, 0);
                 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t][1], B, acc1, 0, 0, 0);
             }
         };
         if (li + 1 < kLayers) {
             const int s = 29 * (li + 1), e = kInfPx - 29 * (li + 1), ngr = (e - s + 15) / 16;
-            for (int gi = wv; gi < ngr; gi += kInfThreads / 64) {
+            for (int gi = wv; gi < ngr; gi += kWaves) {
                 f32x4 acc[2];
                 conv16(s + 16 * gi, e - 1, acc[0], acc[1]);
                 const int f = s + 16 * gi + n;
                 if (f < e) {
                     const bool in = in_image(f);
+                    uint8_t *o = dst + f * kInfStride + 8 * q;
 #pragma unroll
                     for (int nt = 0; nt < 2; nt++) {
-                        uint32_t pk[2];
-#pragma unroll
-                        for (int i = 0; i < 4; i += 2)
-                            pk[i / 2] = in ? ((uint32_t)to_bf16(fmaxf(acc[nt][i] + b[nt][i], 0.f)) |
-                                              ((uint32_t)to_bf16(fmaxf(acc[nt][i + 1] + b[nt][i + 1], 0.f)) << 16))
-                                           : 0u;
-                        *reinterpret_cast<uint2 *>(dst + inf_slot(f, 2 * nt + (q >> 1)) + 8 * (q & 1)) =
-                            make_uint2(pk[0], pk[1]);
+                        const float4 bb = bias4(nt);
+                        const uint32_t lo = pk_bf16(fmaxf(acc[nt][0] + bb.x, 0.f), fmaxf(acc[nt][1] + bb.y, 0.f));
+                        const uint32_t hi = pk_bf16(fmaxf(acc[nt][2] + bb.z, 0.f), fmaxf(acc[nt][3] + bb.w, 0.f));
+                        *reinterpret_cast<uint2 *>(o + 32 * nt) = in ? make_uint2(lo, hi) : make_uint2(0u, 0u);
                     }
                 }
             }
         } else {
             // last layer: inner rows; logits, softmax, probabilities into dst as fp32 [16][16][32]
             float *prob = reinterpret_cast<float *>(dst);
-            for (int r = wv; r < kInfOut; r += kInfThreads / 64) {
+            for (int r = wv; r < kInfOut; r += kWaves) {
                 f32x4 acc[2];
                 const int f0 = (kLayers + r) * kInfR + kLayers;
                 conv16(f0, kInfPx - 1, acc[0], acc[1]);
@@ -595,13 +597,15 @@ __global__ __launch_bounds__(kInfThreads) void k_infer(const uint16_t *__restric
                 float z[2][4];
 #pragma unroll
                 for (int nt = 0; nt < 2; nt++) {
+                    const float4 bb = bias4(nt);
+                    const float b4[4] = {bb.x, bb.y, bb.z, bb.w};
                     const uint32_t o = base == sgmk::kDrop ? base : base + (uint32_t)((16 * nt + 4 * q) * 2);
                     const uint2 fm = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rx, o, 0, 0));
                     const uint16_t fv[4] = {(uint16_t)fm.x, (uint16_t)(fm.x >> 16), (uint16_t)fm.y,
                                             (uint16_t)(fm.y >> 16)};
 #pragma unroll
                     for (int i = 0; i < 4; i++)
-                        z[nt][i] = acc[nt][i] + b[nt][i] - kSkip * from_bf16(fv[i]);
+                        z[nt][i] = acc[nt][i] + b4[i] - kSkip * from_bf16(fv[i]);
                 }
                 float mx = z[0][0];
 #pragma unroll
@@ -633,11 +637,13 @@ __global__ __launch_bounds__(kInfThreads) void k_infer(const uint16_t *__restric
             }
         }
         if (li + 1 < kLayers)
-            commit_layer((li + 1) & 1);
+            commit_layer(li + 1);
         __syncthreads();
     }
 
-    // head: half-wave (tile, lane = bin) as in k_tile_head
+    // head: half-wave (tile, lane = bin) as in k_tile_head; 16 tiles
+    if (tid >= 16 * 32)
+        return;
     const float *prob = reinterpret_cast<const float *>(act[kLayers & 1]);
     const int tw = (w + 3) / 4, th = (h + 3) / 4;
     const int tt = tid >> 5, k = tid & 31;
