@@ -122,11 +122,13 @@ int sgmh_window_dev(sgmh_ctx *ctx, const uint8_t *d_left, const uint8_t *d_right
 
 /* ---- the learned path's window-proposal network (no reference
  *      counterpart): 4x-downsampled matching features, 6 conv layers
- *      3x3 32->32 (bf16 MFMA, fp32 accumulation), per 16x16 tile a softmax over
- *      32 disparity bins of 4 (D = 128), the 32-disparity window (two 16-blocks)
+ *      3x3 32->32 (bf16 MFMA, fp32 accumulation), per low-resolution pixel a
+ *      softmax over 32 disparity bins of 4 (D = 128); per 16x16 tile the mean
+ *      of its pixels' distributions, the 32-disparity window (two 16-blocks)
  *      with the most mass, that mass as the confidence.  Frames need W and H
- *      multiples of 4.  A network owns its parameters (fp32 master copy, bf16
- *      packed copies), Adam state and activation buffers. ---- */
+ *      multiples of 4 and 4-byte aligned images.  A network owns its parameters
+ *      (fp32 master copy, bf16 packed copies), Adam state and activation
+ *      buffers. ---- */
 typedef struct sgmh_cnn sgmh_cnn;
 size_t sgmh_cnn_num_params(void);
 int sgmh_cnn_create(sgmh_ctx *ctx, sgmh_cnn **out);
@@ -139,9 +141,16 @@ int sgmh_cnn_get_params(sgmh_cnn *net, float *theta, size_t n);
  * the best window where its confidence >= thresh, else all 8 blocks. */
 int sgmh_cnn_propose_dev(sgmh_cnn *net, const uint8_t *d_left, const uint8_t *d_right, int W, int H,
                          float thresh, uint8_t *d_blk, float *d_conf, void *stream);
+/* The learned frame in one call: proposal (at `thresh`) and the windowed SGM
+ * over it at D = 128, the census running on a side stream beside the
+ * network.  Same disparities as sgmh_cnn_propose_dev + sgmh_window_dev;
+ * d_blk (may be NULL) receives the block table. */
+int sgmh_learned_dev(sgmh_cnn *net, const uint8_t *d_left, const uint8_t *d_right, int W, int H,
+                     float thresh, uint8_t *d_blk, uint8_t *d_disp, void *stream);
 /* One Adam step (lr > 0) on one frame against target disparities (the plain
  * full-D SGM output); lr <= 0 computes the gradients only.  *loss (may be NULL)
- * receives the tile cross-entropy before the step. */
+ * receives the per-pixel cross-entropy against the histograms of the target's
+ * 4x4 blocks, before the step. */
 int sgmh_cnn_train_step_dev(sgmh_cnn *net, const uint8_t *d_left, const uint8_t *d_right,
                             const uint8_t *d_target, int W, int H, float lr, float *loss, void *stream);
 int sgmh_cnn_get_grads(sgmh_cnn *net, float *grad, size_t n);

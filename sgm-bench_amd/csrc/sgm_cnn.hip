@@ -69,6 +69,13 @@ __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(
 constexpr int kFeatTX = 32, kFeatTY = 8;
 constexpr int kPhases = 4;
 __device__ __forceinline__ uint32_t sum4(uint32_t v, uint32_t acc) { return __builtin_amdgcn_udot4(v, 0x01010101u, acc, false); }
+// |a - b| + acc in one v_sad_u32 (the compiler forms sub, max and add)
+__device__ __forceinline__ uint32_t sad(uint32_t a, uint32_t b, uint32_t acc)
+{
+    uint32_t r;
+    asm("v_sad_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(acc));
+    return r;
+}
 
 __global__ __launch_bounds__(256) void k_features(const uint8_t *__restrict__ Lim, const uint8_t *__restrict__ Rim,
                                                   int W, int H, uint16_t *__restrict__ X)
@@ -79,62 +86,110 @@ __global__ __launch_bounds__(256) void k_features(const uint8_t *__restrict__ Li
     __shared__ int sr[kPhases][NR][RC];     // Rd_ph, columns x0-32 .. x0+32
     const int x0 = blockIdx.x * kFeatTX, y0 = blockIdx.y * kFeatTY;
     const int tid = threadIdx.x;
-    for (int i = tid; i < NR * LC; i += 256) {
+    // Staging: every load unconditional (indices clamped, stores predicated), so
+    // all of a thread's loads are in flight at once.
+    constexpr int LIT = (NR * LC + 255) / 256, RIT = (NR * RC + 255) / 256;
+    uint32_t lw[LIT][4], rhi[RIT][4], rlo[RIT][4];
+#pragma unroll
+    for (int it = 0; it < LIT; it++) {
+        const int i = min(tid + 256 * it, NR * LC - 1);
         const int r = i / LC, c = i - r * LC;
         const int Y = min(max(y0 - 1 + r, 0), h - 1), Xc = min(max(x0 - 1 + c, 0), w - 1);
-        uint32_t acc = 0;
 #pragma unroll
         for (int dy = 0; dy < 4; dy++)
-            acc = sum4(*reinterpret_cast<const uint32_t *>(Lim + (size_t)(4 * Y + dy) * W + 4 * Xc), acc);
-        sl[r][c] = (int)acc;
+            lw[it][dy] = *reinterpret_cast<const uint32_t *>(Lim + (size_t)(4 * Y + dy) * W + 4 * Xc);
     }
-    for (int i = tid; i < NR * RC; i += 256) {
+#pragma unroll
+    for (int it = 0; it < RIT; it++) {
+        const int i = min(tid + 256 * it, NR * RC - 1);
         const int r = i / RC, c = i - r * RC;
         const int Y = min(max(y0 - 1 + r, 0), h - 1), Xc = min(max(x0 - 32 + c, 0), w - 1);
-        uint32_t acc[kPhases] = {0, 0, 0, 0};
 #pragma unroll
         for (int dy = 0; dy < 4; dy++) {
             const uint8_t *row = Rim + (size_t)(4 * Y + dy) * W;
-            const uint32_t hi = *reinterpret_cast<const uint32_t *>(row + 4 * Xc);
-            // columns 4Xc-4 .. 4Xc-1; left of the image they repeat column 0
-            const uint32_t lo = Xc > 0 ? *reinterpret_cast<const uint32_t *>(row + 4 * Xc - 4) : (uint32_t)row[0] * 0x01010101u;
+            rhi[it][dy] = *reinterpret_cast<const uint32_t *>(row + 4 * Xc);
+            rlo[it][dy] = *reinterpret_cast<const uint32_t *>(row + 4 * max(Xc - 1, 0));
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < LIT; it++) {
+        const int i = tid + 256 * it;
+        uint32_t acc = 0;
+#pragma unroll
+        for (int dy = 0; dy < 4; dy++)
+            acc = sum4(lw[it][dy], acc);
+        if (i < NR * LC)
+            sl[i / LC][i % LC] = (int)acc;
+    }
+#pragma unroll
+    for (int it = 0; it < RIT; it++) {
+        const int i = tid + 256 * it;
+        const int r = min(i, NR * RC - 1) / RC, c = min(i, NR * RC - 1) % RC;
+        const bool edge = x0 - 32 + c <= 0; // columns 4Xc-4 .. 4Xc-1 left of the image repeat column 0
+        uint32_t acc[kPhases] = {0, 0, 0, 0};
+#pragma unroll
+        for (int dy = 0; dy < 4; dy++) {
+            const uint32_t hi = rhi[it][dy];
+            const uint32_t lo = edge ? (hi & 0xFFu) * 0x01010101u : rlo[it][dy];
             acc[0] = sum4(hi, acc[0]);
 #pragma unroll
             for (int ph = 1; ph < kPhases; ph++)
                 acc[ph] = sum4(__builtin_amdgcn_alignbyte(hi, lo, 4 - ph), acc[ph]);
         }
+        if (i < NR * RC) {
 #pragma unroll
-        for (int ph = 0; ph < kPhases; ph++)
-            sr[ph][r][c] = (int)acc[ph];
+            for (int ph = 0; ph < kPhases; ph++)
+                sr[ph][r][c] = (int)acc[ph];
+        }
     }
     __syncthreads();
     const int k = tid & 31, cg = tid >> 5;
+    // interior workgroups: no window column is clamped and x' - k >= 0 for every bin
+    const bool interior = x0 >= 32 && x0 + kFeatTX < w;
     for (int xl = cg; xl < kFeatTX; xl += 8) {
         const int x = x0 + xl;
         if (x >= w)
             break;
-        int cx[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-            cx[j] = min(max(x - 1 + j, 0), w - 1);
         int best[kFeatTY];
 #pragma unroll
         for (int y = 0; y < kFeatTY; y++)
             best[y] = 0x7FFFFFFF;
+        if (interior) {
 #pragma unroll
-        for (int ph = 0; ph < kPhases; ph++) {
-            int h3[NR];
+            for (int ph = 0; ph < kPhases; ph++) {
+                int h3[NR];
 #pragma unroll
-            for (int r = 0; r < NR; r++) {
-                int acc = 0;
+                for (int r = 0; r < NR; r++) {
+                    uint32_t acc = 0;
 #pragma unroll
-                for (int j = 0; j < 3; j++)
-                    acc += cx[j] - k >= 0 ? abs(sl[r][cx[j] - x0 + 1] - sr[ph][r][cx[j] - k - x0 + 32]) : 4080;
-                h3[r] = acc;
+                    for (int j = 0; j < 3; j++)
+                        acc = sad((uint32_t)sl[r][xl + j], (uint32_t)sr[ph][r][xl + j - k + 31], acc);
+                    h3[r] = (int)acc;
+                }
+#pragma unroll
+                for (int y = 0; y < kFeatTY; y++)
+                    best[y] = min(best[y], h3[y] + h3[y + 1] + h3[y + 2]);
             }
+        } else {
+            int cx[3];
 #pragma unroll
-            for (int y = 0; y < kFeatTY; y++)
-                best[y] = min(best[y], h3[y] + h3[y + 1] + h3[y + 2]);
+            for (int j = 0; j < 3; j++)
+                cx[j] = min(max(x - 1 + j, 0), w - 1);
+#pragma unroll
+            for (int ph = 0; ph < kPhases; ph++) {
+                int h3[NR];
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    int acc = 0;
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        acc += cx[j] - k >= 0 ? abs(sl[r][cx[j] - x0 + 1] - sr[ph][r][cx[j] - k - x0 + 32]) : 4080;
+                    h3[r] = acc;
+                }
+#pragma unroll
+                for (int y = 0; y < kFeatTY; y++)
+                    best[y] = min(best[y], h3[y] + h3[y + 1] + h3[y + 2]);
+            }
         }
 #pragma unroll
         for (int y = 0; y < kFeatTY; y++)
@@ -732,6 +787,9 @@ struct sgmh_cnn {
     float *loss = nullptr;
     uint8_t *blk = nullptr;
     float *conf = nullptr;
+    // the learned frame's census runs on `side` beside the proposal
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -892,6 +950,12 @@ void sgmh_cnn_destroy(sgmh_cnn *c)
     for (auto a : c->act)
         if (a)
             (void)hipFree(a);
+    if (c->ev_fork)
+        (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join)
+        (void)hipEventDestroy(c->ev_join);
+    if (c->side)
+        (void)hipStreamDestroy(c->side);
     delete c;
 }
 
@@ -934,7 +998,44 @@ int sgmh_cnn_propose_dev(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W,
     return 0;
 }
 
-// One optimisation step on one frame: forward, tile cross-entropy against the
+// The learned frame: census of both images on a side stream, forked from
+// `stream`, while the proposal network runs on `stream`; then the windowed
+// paths and argmin over the proposal.  Same result as sgmh_cnn_propose_dev
+// followed by sgmh_window_dev at D = 128.
+int sgmh_learned_dev(sgmh_cnn *c, const uint8_t *l, const uint8_t *r, int W, int H, float thresh,
+                     uint8_t *d_blk, uint8_t *d_disp, void *stream)
+{
+    REQUIRE(l && r && d_disp, "sgmh_learned_dev: NULL argument");
+    if (cnn_check(c, W, H))
+        return -1;
+    sgmh_ctx *ctx = c->ctx;
+    hipStream_t s = (hipStream_t)stream;
+    if (!c->side) {
+        CK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        CK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        CK(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
+    if (window_prepare(ctx, W, H, kDisp))
+        return -1;
+    begin_call(ctx, s);
+    CK(hipEventRecord(c->ev_fork, s));
+    CK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    if (window_census(ctx, c->side, l, r, W, H))
+        return -1;
+    CK(hipEventRecord(c->ev_join, c->side));
+    uint8_t *blk = d_blk ? d_blk : c->blk;
+    if (cnn_features(c, l, r, W, H, s))
+        return -1;
+    const int w = W / 4, h = H / 4;
+    hipLaunchKernelGGL(k_infer, dim3(((w + kInfOut - 1) / kInfOut) * ((h + kInfOut - 1) / kInfOut)),
+                       dim3(kInfThreads), 0, s, (const uint16_t *)c->act[0], w, h, (const uint16_t *)c->Wf,
+                       (const float *)c->theta, thresh, blk, c->conf);
+    CK(hipGetLastError());
+    CK(hipStreamWaitEvent(s, c->ev_join, 0));
+    return window_rest(ctx, s, W, H, kDisp, blk, d_disp);
+}
+
+// One optimisation step on one frame: forward, per-pixel cross-entropy against the
 // target disparities, backward, Adam.  *loss (host) receives the loss before
 // the update (this synchronises the stream).  lr <= 0: gradients only, kept in
 // the network's gradient buffer (sgmh_cnn_get_grads), no update.

@@ -343,21 +343,42 @@ static int plain_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const u
     return 0;
 }
 
-static int window_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W, int H,
-                          int D, const uint8_t *blk, uint8_t *disp)
+// The windowed path in two parts, so that the learned path can run the census
+// beside the proposal network: window_prepare sizes the workspaces (it may
+// allocate and synchronise, so it runs before any fork), window_census fills
+// the context's codes, window_rest runs the paths and the argmin over them.
+int window_prepare(sgmh_ctx *ctx, int W, int H, int D)
 {
     const size_t N = (size_t)W * H;
     REQUIRE(N * D < (1ull << 31) && N * 32 < (1ull << 31), "sgmh_window: %dx%d D=%d exceeds the 2 GiB per-volume limit", W, H, D);
     if (ensure_codes(ctx->cl, N) || ensure_codes(ctx->cr, N) || ensure(ctx->vol, N * (D + 32) * 4))
         return -1;
+    return 0;
+}
+
+int window_census(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W, int H)
+{
+    uint64_t *cl = (uint64_t *)ctx->cl.p, *cr = (uint64_t *)ctx->cr.p;
+    PhaseScope ps(ctx, s, PH_CENSUS);
+    if (launch_census(s, l, cl + kLead, W, H, W) || launch_census(s, r, cr + kLead, W, H, W))
+        return -1;
+    return 0;
+}
+
+static int window_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W, int H,
+                          int D, const uint8_t *blk, uint8_t *disp)
+{
+    if (window_prepare(ctx, W, H, D) || window_census(ctx, s, l, r, W, H))
+        return -1;
+    return window_rest(ctx, s, W, H, D, blk, disp);
+}
+
+int window_rest(sgmh_ctx *ctx, hipStream_t s, int W, int H, int D, const uint8_t *blk, uint8_t *disp)
+{
+    const size_t N = (size_t)W * H;
     uint64_t *cl = (uint64_t *)ctx->cl.p, *cr = (uint64_t *)ctx->cr.p;
     uint8_t *vol = (uint8_t *)ctx->vol.p;
     uint8_t *nvol = vol + N * D * 4; // narrow volumes, [N][32] per direction
-    {
-        PhaseScope ps(ctx, s, PH_CENSUS);
-        if (launch_census(s, l, cl + kLead, W, H, W) || launch_census(s, r, cr + kLead, W, H, W))
-            return -1;
-    }
     WinArgs a{};
     a.cl = cl;
     a.cr = cr;

@@ -110,6 +110,11 @@ struct PhaseScope {
     }
 };
 
+// the windowed path's parts (sgm_hip.hip), for the learned path's entry point
+int window_prepare(sgmh_ctx *ctx, int W, int H, int D);
+int window_census(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W, int H);
+int window_rest(sgmh_ctx *ctx, hipStream_t s, int W, int H, int D, const uint8_t *blk, uint8_t *disp);
+
 inline void begin_call(sgmh_ctx *ctx, hipStream_t s)
 {
     ctx->ev_used = 0;

@@ -121,6 +121,7 @@ def hip_lib() -> ctypes.CDLL:
         h.sgmh_cnn_set_params.argtypes = [_P, _P, ctypes.c_size_t]
         h.sgmh_cnn_get_params.argtypes = [_P, _P, ctypes.c_size_t]
         h.sgmh_cnn_propose_dev.argtypes = [_P, _P, _P, _I, _I, ctypes.c_float, _P, _P, _P]
+        h.sgmh_learned_dev.argtypes = [_P, _P, _P, _I, _I, ctypes.c_float, _P, _P, _P]
         h.sgmh_cnn_train_step_dev.argtypes = [_P, _P, _P, _P, _I, _I, ctypes.c_float,
                                               ctypes.POINTER(ctypes.c_float), _P]
         h.sgmh_cnn_get_grads.argtypes = [_P, _P, ctypes.c_size_t]
@@ -137,7 +138,7 @@ def hip_lib() -> ctypes.CDLL:
                      "sgmh_batch_plan", "sgmh_batch_run", "sgmh_batch_run_loop",
                      "sgmh_batch_describe", "sgmh_window", "sgmh_window_dev",
                      "sgmh_cnn_create", "sgmh_cnn_set_params", "sgmh_cnn_get_params",
-                     "sgmh_cnn_propose_dev", "sgmh_cnn_train_step_dev", "sgmh_cnn_get_grads",
+                     "sgmh_cnn_propose_dev", "sgmh_learned_dev", "sgmh_cnn_train_step_dev", "sgmh_cnn_get_grads",
                      "sgmh_cnn_get_activations"):
             getattr(h, name).restype = _I
         _hip = h
@@ -452,6 +453,12 @@ class Cnn:
                     d_conf: int = 0, stream: int = 0) -> None:
         self._ctx._ck(self._lib.sgmh_cnn_propose_dev(self._h, d_left, d_right, W, H, thresh, d_blk,
                                                       d_conf or None, stream or None))
+
+    def learned_dev(self, d_left: int, d_right: int, W: int, H: int, thresh: float, d_disp: int,
+                    d_blk: int = 0, stream: int = 0) -> None:
+        """Proposal + windowed SGM at D=128 in one call (census beside the network)."""
+        self._ctx._ck(self._lib.sgmh_learned_dev(self._h, d_left, d_right, W, H, thresh, d_blk or None,
+                                                  d_disp, stream or None))
 
     def train_step_dev(self, d_left: int, d_right: int, d_target: int, W: int, H: int, lr: float,
                        stream: int = 0) -> float:

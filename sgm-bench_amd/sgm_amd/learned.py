@@ -235,6 +235,7 @@ def main():
     ap.add_argument("--eval", type=int, default=8, help="evaluation frames")
     ap.add_argument("--out", default=MODEL_PATH)
     ap.add_argument("--eval-only", action="store_true")
+    ap.add_argument("--thresh", default="0.9,0.8,0.95", help="comma-separated confidence thresholds to evaluate")
     a = ap.parse_args()
     ctx = Context(0)
     net = Cnn(ctx, load_params() if a.eval_only else init_params())
@@ -243,7 +244,7 @@ def main():
         os.makedirs(os.path.dirname(a.out), exist_ok=True)
         np.save(a.out, net.params(), allow_pickle=False)
         print("saved", a.out)
-    for th in (0.9, 0.8, 0.95):
+    for th in (float(t) for t in a.thresh.split(",")):
         print(json.dumps(evaluate(ctx, net, range(EVAL_SEED0, EVAL_SEED0 + a.eval), th)), flush=True)
 
 

@@ -46,6 +46,29 @@ def test_learned_path_is_windowed_sgm_on_the_proposal(ctx):
     np.testing.assert_array_equal(ctx.window(L, R, blk, D), oracle.sgm_window(L, R, blk, D, 4))
 
 
+def test_learned_call_equals_proposal_then_window(ctx):
+    """The one-call learned frame (census on a side stream beside the network)
+    gives the block table and disparities of the two separate calls."""
+    L, R = learned.scene(learned.EVAL_SEED0 + 3, 640, 480)
+    h, w = L.shape
+    net = sgm_amd.Cnn(ctx, learned.load_params())
+    T = ((h + 15) // 16) * ((w + 15) // 16)
+    dl, dr = dev(L), dev(R)
+    b1 = torch.empty(T, dtype=torch.uint8, device="cuda")
+    b2 = torch.empty_like(b1)
+    d1 = torch.empty((h, w), dtype=torch.uint8, device="cuda")
+    d2 = torch.empty_like(d1)
+    s = torch.cuda.current_stream().cuda_stream
+    for _ in range(3):   # repeated calls reuse the side stream and events
+        net.propose_dev(dl.data_ptr(), dr.data_ptr(), w, h, 0.9, b1.data_ptr(), 0, s)
+        ctx.window_dev(dl.data_ptr(), dr.data_ptr(), w, h, D, b1.data_ptr(), d1.data_ptr(), s)
+        net.learned_dev(dl.data_ptr(), dr.data_ptr(), w, h, 0.9, d2.data_ptr(), b2.data_ptr(), s)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(b2.cpu().numpy(), b1.cpu().numpy())
+        np.testing.assert_array_equal(d2.cpu().numpy(), d1.cpu().numpy())
+    net.close()
+
+
 @pytest.fixture(scope="module")
 def frames():
     return [learned.scene(learned.EVAL_SEED0 + i, W, H) for i in range(3)]
