@@ -69,23 +69,25 @@ __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uin
     for (int p = 0; p < 4; p++) {
         const uint32_t centre = (win[3][(p + 4) >> 2] >> (8 * ((p + 4) & 3))) & 0xFFu;
         uint32_t lo = 0, hi = 0;
-        int n = 0;
         // bit n: n-th neighbour in (dy outer -3..3, dx inner -4..4) order,
-        // centre skipped (common/sgm_params.h:45-47)
+        // centre skipped (common/sgm_params.h:45-47).  The neighbours are
+        // taken in reverse and shifted in at the bottom, so bit n lands at
+        // position n: nb - centre has bit 31 set iff nb < centre, and
+        // v_alignbit(word, d, 31) = word << 1 | d >> 31.
+        int n = 61;
 #pragma unroll
-        for (int r = 0; r < 7; r++)
+        for (int r = 6; r >= 0; r--)
 #pragma unroll
-            for (int c = 0; c < 9; c++) {
+            for (int c = 8; c >= 0; c--) {
                 if (r == 3 && c == 4)
                     continue;
                 const int b = p + c; // byte index of (x+p + c-4) in the window
-                const uint32_t nb = (win[r][b >> 2] >> (8 * (b & 3))) & 0xFFu;
-                const uint32_t bit = nb < centre ? 1u : 0u;
-                if (n < 32)
-                    lo |= bit << n;
+                const uint32_t d = ((win[r][b >> 2] >> (8 * (b & 3))) & 0xFFu) - centre;
+                if (n >= 32)
+                    hi = __builtin_amdgcn_alignbit(hi, d, 31);
                 else
-                    hi |= bit << (n - 32);
-                n++;
+                    lo = __builtin_amdgcn_alignbit(lo, d, 31);
+                n--;
             }
         code[p] = ((uint64_t)hi << 32) | lo;
     }

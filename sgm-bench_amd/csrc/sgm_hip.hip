@@ -39,12 +39,15 @@ void sgmh_set_error(const char *fmt, ...)
 // ---------------------------------------------------------------------------
 // kernels: one grid point per tile / pixel group / line group
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_census(const uint8_t *__restrict__ img,
-                                                uint64_t *__restrict__ out, int W, int H,
-                                                int ostride)
+// The census of a stereo pair in one launch: blockIdx.z = 0 the left image, 1 the right.
+__global__ __launch_bounds__(256) void k_census(const uint8_t *__restrict__ img0, const uint8_t *__restrict__ img1,
+                                                uint64_t *__restrict__ out0, uint64_t *__restrict__ out1, int W,
+                                                int H, int ostride)
 {
     __shared__ uint32_t tile[kCenRows * kCenRowWords];
-    census_tile(img, out, W, H, ostride, blockIdx.x * kCenTX, blockIdx.y * kCenTY, tile);
+    const bool second = blockIdx.z != 0;
+    census_tile(second ? img1 : img0, second ? out1 : out0, W, H, ostride, blockIdx.x * kCenTX,
+                blockIdx.y * kCenTY, tile);
 }
 
 __global__ __launch_bounds__(256) void k_half(const uint8_t *__restrict__ img, int W, int H,
@@ -160,11 +163,12 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, u
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static int launch_census(hipStream_t s, const uint8_t *img, uint64_t *out, int W, int H,
-                         int ostride)
+// census of one image, or of a pair (img1 != nullptr) in one launch
+static int launch_census(hipStream_t s, const uint8_t *img0, uint64_t *out0, int W, int H, int ostride,
+                         const uint8_t *img1 = nullptr, uint64_t *out1 = nullptr)
 {
-    dim3 grid((W + kCenTX - 1) / kCenTX, (H + kCenTY - 1) / kCenTY);
-    hipLaunchKernelGGL(k_census, grid, dim3(256), 0, s, img, out, W, H, ostride);
+    dim3 grid((W + kCenTX - 1) / kCenTX, (H + kCenTY - 1) / kCenTY, img1 ? 2 : 1);
+    hipLaunchKernelGGL(k_census, grid, dim3(256), 0, s, img0, img1, out0, out1, W, H, ostride);
     CK(hipGetLastError());
     return 0;
 }
@@ -321,7 +325,7 @@ static int plain_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const u
     uint8_t *vol = (uint8_t *)ctx->vol.p;
     {
         PhaseScope ps(ctx, s, PH_CENSUS);
-        if (launch_census(s, l, cl + kLead, W, H, W) || launch_census(s, r, cr + kLead, W, H, W))
+        if (launch_census(s, l, cl + kLead, W, H, W, r, cr + kLead))
             return -1;
     }
     {
@@ -360,7 +364,7 @@ int window_census(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t 
 {
     uint64_t *cl = (uint64_t *)ctx->cl.p, *cr = (uint64_t *)ctx->cr.p;
     PhaseScope ps(ctx, s, PH_CENSUS);
-    if (launch_census(s, l, cl + kLead, W, H, W) || launch_census(s, r, cr + kLead, W, H, W))
+    if (launch_census(s, l, cl + kLead, W, H, W, r, cr + kLead))
         return -1;
     return 0;
 }
@@ -453,12 +457,12 @@ static int ms_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint
         hipLaunchKernelGGL(k_sobel, dim3((hw + 63) / 64, (hh + 3) / 4), dim3(256), 0, s, hl, hw, hh, sl);
         hipLaunchKernelGGL(k_sobel, dim3((hw + 63) / 64, (hh + 3) / 4), dim3(256), 0, s, hr, hw, hh, sr);
         CK(hipGetLastError());
-        if (launch_census(s, sl, cl1, hw, hh, hw) || launch_census(s, sr, cr1, hw, hh, hw))
+        if (launch_census(s, sl, cl1, hw, hh, hw, sr, cr1))
             return -1;
         hipLaunchKernelGGL(k_sobel, gf, dim3(256), 0, s, l, W, H, sl);
         hipLaunchKernelGGL(k_sobel, gf, dim3(256), 0, s, r, W, H, sr);
         CK(hipGetLastError());
-        if (launch_census(s, sl, cl2, W, H, W) || launch_census(s, sr, cr2, W, H, W))
+        if (launch_census(s, sl, cl2, W, H, W, sr, cr2))
             return -1;
     }
     {

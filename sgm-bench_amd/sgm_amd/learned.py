@@ -51,9 +51,10 @@ def load_params() -> np.ndarray:
 
 
 def train(ctx: Context, net: Cnn, n_pairs: int = 2000, epochs: int = 1, lr: float = 2e-3,
-          W: int = 1920, H: int = 1080, chunk: int = 100, log=print) -> List[float]:
-    """Adam on the tile cross-entropy; every pair is generated, labelled by the
-    plain full-D SGM on the GPU and used once per epoch."""
+          W: int = 1920, H: int = 1080, chunk: int = 100, log=print, first_pair: int = 0) -> List[float]:
+    """Adam on the per-pixel cross-entropy; every pair (seeds TRAIN_SEED0 +
+    first_pair ...) is generated, labelled by the plain full-D SGM on the GPU
+    and used once per epoch."""
     import torch
     s = torch.cuda.current_stream().cuda_stream
     dl = torch.empty((H, W), dtype=torch.uint8, device="cuda")
@@ -65,7 +66,8 @@ def train(ctx: Context, net: Cnn, n_pairs: int = 2000, epochs: int = 1, lr: floa
     total = n_pairs * epochs
     for ep in range(epochs):
         for c0 in range(0, n_pairs, chunk):
-            batch = scenes(range(TRAIN_SEED0 + c0, TRAIN_SEED0 + min(n_pairs, c0 + chunk)), W, H)
+            batch = scenes(range(TRAIN_SEED0 + first_pair + c0, TRAIN_SEED0 + first_pair + min(n_pairs, c0 + chunk)),
+                           W, H)
             for L, R in batch:
                 dl.copy_(torch.from_numpy(L))
                 dr.copy_(torch.from_numpy(R))
@@ -235,12 +237,14 @@ def main():
     ap.add_argument("--eval", type=int, default=8, help="evaluation frames")
     ap.add_argument("--out", default=MODEL_PATH)
     ap.add_argument("--eval-only", action="store_true")
+    ap.add_argument("--resume", action="store_true", help="train on from the committed model")
+    ap.add_argument("--first-pair", type=int, default=0, help="training seeds start at TRAIN_SEED0 + this")
     ap.add_argument("--thresh", default="0.9,0.8,0.95", help="comma-separated confidence thresholds to evaluate")
     a = ap.parse_args()
     ctx = Context(0)
-    net = Cnn(ctx, load_params() if a.eval_only else init_params())
+    net = Cnn(ctx, load_params() if (a.eval_only or a.resume) else init_params())
     if not a.eval_only:
-        train(ctx, net, a.pairs, a.epochs, a.lr)
+        train(ctx, net, a.pairs, a.epochs, a.lr, first_pair=a.first_pair)
         os.makedirs(os.path.dirname(a.out), exist_ok=True)
         np.save(a.out, net.params(), allow_pickle=False)
         print("saved", a.out)
