@@ -171,20 +171,32 @@ __global__ __launch_bounds__(256) void k_features(const uint8_t *__restrict__ Li
                     best[y] = min(best[y], h3[y] + h3[y + 1] + h3[y + 2]);
             }
         } else {
-            int cx[3];
+            // edge workgroups: clamped window columns, and bins whose x' - k < 0
+            // cost 4080 -- fixed per (column, bin), so that column's term is
+            // made 0 (R := L) and the 4080s start the row sum
+            int li[3], ri[3];
+            bool valid[3];
+            uint32_t base = 0;
 #pragma unroll
-            for (int j = 0; j < 3; j++)
-                cx[j] = min(max(x - 1 + j, 0), w - 1);
+            for (int j = 0; j < 3; j++) {
+                const int cx = min(max(x - 1 + j, 0), w - 1);
+                li[j] = cx - x0 + 1;
+                ri[j] = cx - k - x0 + 32; // in the staged range for every k (cx >= x0 - 1)
+                valid[j] = cx - k >= 0;
+                base += valid[j] ? 0u : 4080u;
+            }
 #pragma unroll
             for (int ph = 0; ph < kPhases; ph++) {
                 int h3[NR];
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
-                    int acc = 0;
+                    uint32_t acc = base;
 #pragma unroll
-                    for (int j = 0; j < 3; j++)
-                        acc += cx[j] - k >= 0 ? abs(sl[r][cx[j] - x0 + 1] - sr[ph][r][cx[j] - k - x0 + 32]) : 4080;
-                    h3[r] = acc;
+                    for (int j = 0; j < 3; j++) {
+                        const uint32_t L = (uint32_t)sl[r][li[j]], R = (uint32_t)sr[ph][r][ri[j]];
+                        acc = sad(L, valid[j] ? R : L, acc);
+                    }
+                    h3[r] = (int)acc;
                 }
 #pragma unroll
                 for (int y = 0; y < kFeatTY; y++)
@@ -525,6 +537,7 @@ constexpr int kInfPx = kInfR * kInfR;        // 784
 constexpr int kInfStride = 80;               // bytes per region pixel
 constexpr int kInfThreads = 1024;
 constexpr int kWBytes = kC * 288 * 2;        // one layer's packed weights
+constexpr int kWRow = 288 * 2 + 16;          // LDS bytes per output channel: 16-byte reads of 16 channels hit 16 bank groups
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -540,7 +553,7 @@ __global__ __launch_bounds__(kInfThreads) void k_infer(const uint16_t *__restric
                                                        uint8_t *__restrict__ blk, float *__restrict__ conf)
 {
     __shared__ __align__(16) uint8_t act[2][kInfPx * kInfStride];
-    __shared__ __align__(16) uint8_t wts[kWBytes];
+    __shared__ __align__(16) uint8_t wts[kC * kWRow];
     __shared__ float bsh[2][kC];
     typedef uint32_t v4u __attribute__((ext_vector_type(4)));
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, n = l & 15, q = l >> 4;
@@ -578,7 +591,7 @@ __global__ __launch_bounds__(kInfThreads) void k_infer(const uint16_t *__restric
         for (int u = 0; u < kWPerThread; u++) {
             const int c = tid + kInfThreads * u;
             if (c < kWChunks)
-                *reinterpret_cast<v4u *>(&wts[16 * c]) = pw[u];
+                *reinterpret_cast<v4u *>(&wts[(c / 36) * kWRow + 16 * (c % 36)]) = pw[u];
         }
         if (tid < kC)
             bsh[li & 1][tid] = pb;
@@ -602,7 +615,7 @@ __global__ __launch_bounds__(kInfThreads) void k_infer(const uint16_t *__restric
         for (int t = 0; t < 9; t++)
 #pragma unroll
             for (int nt = 0; nt < 2; nt++)
-                A[t][nt] = *reinterpret_cast<const bf16x8 *>(wts + ((16 * nt + n) * 288 + 32 * t + 8 * q) * 2);
+                A[t][nt] = *reinterpret_cast<const bf16x8 *>(wts + (16 * nt + n) * kWRow + (32 * t + 8 * q) * 2);
         // this lane's 8 biases, read from LDS where they are used (registers are the limit)
         auto bias4 = [&](int nt) { return *reinterpret_cast<const float4 *>(&bsh[li & 1][16 * nt + 4 * q]); };
         if (li + 1 < kLayers)

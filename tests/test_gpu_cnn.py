@@ -12,6 +12,9 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 SIZES = [(64, 96), (52, 100), (120, 200)]   # (H, W): ragged conv tiles and partial 16x16 tiles
+# frames wide enough for workgroups away from both image edges (the features
+# kernel's unclamped interior path) as well as edge ones
+WIDE = SIZES + [(64, 512)]
 
 
 def case(H, W, seed):
@@ -40,7 +43,7 @@ def grads_on_gpu(ctx, theta, L, R, disp):
     return loss, f, z, g
 
 
-@pytest.mark.parametrize("H,W", SIZES)
+@pytest.mark.parametrize("H,W", WIDE)
 def test_features_bit_exact(ctx, H, W):
     L, R, disp, theta = case(H, W, H + W)
     _, f, _, _ = grads_on_gpu(ctx, theta, L, R, disp)
@@ -81,7 +84,7 @@ def test_gradients_match_oracle(ctx, H, W):
             assert rel < 3e-2 and cos > 0.999, (layer, lo, rel, cos)
 
 
-@pytest.mark.parametrize("H,W", SIZES)
+@pytest.mark.parametrize("H,W", WIDE)
 def test_proposal_matches_oracle(ctx, H, W):
     L, R, disp, theta = case(H, W, H + W + 3)
     # a confident network: sharpen the last layer so both branches of the rule occur

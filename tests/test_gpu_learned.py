@@ -1,9 +1,10 @@
 """The learned window path end to end: the windowed SGM runs exactly on the
 network's proposal, and training is what makes the windows good -- the
 trained network's windows differ from plain D=128 on far fewer pixels than
-an untrained network's at equal full-D fallback rates.  The same proposal
-rule without a network (the per-pixel argmin of the matching features) is
-measured beside them and printed."""
+an untrained network's at equal full-D fallback rates, and (at 30 %
+fallback) on fewer than the same proposal rule without a network (the
+per-pixel argmin of the matching features), which is measured beside them
+and printed."""
 import numpy as np
 import pytest
 
@@ -98,3 +99,5 @@ def test_trained_network_gains(ctx, frames, fallback):
     print(f"fallback {fallback:.0%}: trained {t:.3f}%, untrained {u:.3f}%, without network {hres:.3f}%")
     assert t < 0.5 * u, (t, u)
     assert t < 3.0, t
+    if fallback >= 0.3:   # where the margin is clear, the network also beats the rule without it
+        assert t < hres, (t, hres)
