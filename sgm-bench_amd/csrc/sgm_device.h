@@ -1197,11 +1197,13 @@ __device__ __forceinline__ uint32_t wta_block_key(const uint8_t *__restrict__ sr
     return key;
 }
 
+// `part` (0..3): this wave's quarter of the tile's passes, so a tile is spread
+// over the 4 waves of a workgroup and no wave runs a whole full-D tile alone.
 template <int NB>
 __device__ __forceinline__ void wta_window_tile(const uint8_t *__restrict__ L, size_t dir_stride,
                                                 const uint8_t *__restrict__ Ln, size_t nstride, int ndirs,
                                                 const uint8_t *__restrict__ blk, int tiles_x, int W, int H,
-                                                uint8_t *__restrict__ disp, int t)
+                                                uint8_t *__restrict__ disp, int t, int part)
 {
     constexpr int D = 16 * NB;
     const int lane = threadIdx.x & 63;
@@ -1211,8 +1213,8 @@ __device__ __forceinline__ void wta_window_tile(const uint8_t *__restrict__ L, s
     if (b1 - b0 == 1) {
         const int b = b0 + (lane & 1);
         const bool act = b <= b1;
-#pragma unroll 2
-        for (int it = 0; it < 8; it++) {
+#pragma unroll
+        for (int it = 2 * part; it < 2 * part + 2; it++) {
             const int r = it * 32 + (lane >> 1), y = ty * 16 + (r >> 4), x = tx * 16 + (r & 15);
             const bool valid = y < H && x < W;
             const size_t p = valid ? (size_t)y * W + x : 0;
@@ -1224,11 +1226,11 @@ __device__ __forceinline__ void wta_window_tile(const uint8_t *__restrict__ L, s
                 disp[p] = (uint8_t)(key & 0xFFu);
         }
     } else {
-        constexpr int PPW = 64 / NB;
+        constexpr int PPW = 64 / NB, PASSES = 256 / PPW / 4; // passes per part
         const int sub = lane % NB;
         const bool act = sub >= b0 && sub <= b1;
 #pragma unroll 2
-        for (int it = 0; it < 256 / PPW; it++) {
+        for (int it = PASSES * part; it < PASSES * part + PASSES; it++) {
             const int r = it * PPW + lane / NB, y = ty * 16 + (r >> 4), x = tx * 16 + (r & 15);
             const bool valid = y < H && x < W;
             const size_t p = valid ? (size_t)y * W + x : 0;

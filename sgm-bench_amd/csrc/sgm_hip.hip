@@ -142,9 +142,9 @@ __global__ __launch_bounds__(256) void k_wta_window(const uint8_t *__restrict__ 
                                                     const uint8_t *__restrict__ blk, int tiles_x, int ntiles, int W,
                                                     int H, uint8_t *__restrict__ disp)
 {
-    const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
-    for (int t = wave; t < ntiles; t += (int)gridDim.x * 4)
-        wta_window_tile<NB>(L, dir_stride, Ln, nstride, ndirs, blk, tiles_x, W, H, disp, t);
+    // one workgroup per tile, wave w takes the w-th quarter of its passes
+    for (int t = (int)blockIdx.x; t < ntiles; t += (int)gridDim.x)
+        wta_window_tile<NB>(L, dir_stride, Ln, nstride, ndirs, blk, tiles_x, W, H, disp, t, (int)(threadIdx.x >> 6));
 }
 
 // ---------------------------------------------------------------------------
@@ -418,7 +418,7 @@ int window_rest(sgmh_ctx *ctx, hipStream_t s, int W, int H, int D, const uint8_t
     {
         PhaseScope ps(ctx, s, PH_ARGMIN);
         const int ntiles = a.tiles_x * ((H + 15) / 16);
-        const int wb = (ntiles + 3) / 4;
+        const int wb = ntiles;
         switch (D / 16) {
         case 2: hipLaunchKernelGGL(k_wta_window<2>, dim3(wb), dim3(256), 0, s, vol, N * D, nvol, N * 32, 4, blk, a.tiles_x, ntiles, W, H, disp); break;
         case 4: hipLaunchKernelGGL(k_wta_window<4>, dim3(wb), dim3(256), 0, s, vol, N * D, nvol, N * 32, 4, blk, a.tiles_x, ntiles, W, H, disp); break;
