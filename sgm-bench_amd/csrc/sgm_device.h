@@ -633,9 +633,13 @@ __device__ __forceinline__ void wta_batch(const uint8_t *__restrict__ L, size_t 
         pix[u] = p;
         const int pc = p < npix ? p : npix - 1;
         const uint8_t *src = L + (size_t)pc * D + sub * 16;
+        // each volume byte is read exactly once: stream it past the caches
+        typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 #pragma unroll
-        for (int k = 0; k < NDIRS; k++)
-            v[u][k] = *reinterpret_cast<const uint4 *>(src + k * dir_stride);
+        for (int k = 0; k < NDIRS; k++) {
+            const v4u q = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(src + k * dir_stride));
+            v[u][k] = make_uint4(q[0], q[1], q[2], q[3]);
+        }
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
@@ -1178,9 +1182,11 @@ __device__ __forceinline__ uint32_t wta_block_key(const uint8_t *__restrict__ sr
                                                   uint32_t d0)
 {
     uint32_t ev[4] = {0, 0, 0, 0}, od[4] = {0, 0, 0, 0};
+    typedef uint32_t v4u __attribute__((ext_vector_type(4)));
     for (int k = 0; k < ndirs; k++) {
-        const uint4 q = *reinterpret_cast<const uint4 *>(src + k * dir_stride);
-        const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+        // read once: stream past the caches
+        const v4u q = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(src + k * dir_stride));
+        const uint32_t w[4] = {q[0], q[1], q[2], q[3]};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             ev[i] += w[i] & 0x00FF00FFu;

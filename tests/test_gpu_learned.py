@@ -47,10 +47,11 @@ def test_learned_path_is_windowed_sgm_on_the_proposal(ctx):
     np.testing.assert_array_equal(ctx.window(L, R, blk, D), oracle.sgm_window(L, R, blk, D, 4))
 
 
-def test_learned_call_equals_proposal_then_window(ctx):
+@pytest.mark.parametrize("w,h", [(640, 480), (644, 484)])   # the second: partial 16x16 tiles
+def test_learned_call_equals_proposal_then_window(ctx, w, h):
     """The one-call learned frame (census on a side stream beside the network)
     gives the block table and disparities of the two separate calls."""
-    L, R = learned.scene(learned.EVAL_SEED0 + 3, 640, 480)
+    L, R = learned.scene(learned.EVAL_SEED0 + 3, w, h)
     h, w = L.shape
     net = sgm_amd.Cnn(ctx, learned.load_params())
     T = ((h + 15) // 16) * ((w + 15) // 16)
