@@ -184,31 +184,41 @@ __device__ __forceinline__ uint64_t ld64(rsrc_t r, uint32_t off)
     return ((uint64_t)v[1] << 32) | v[0];
 }
 
-// Store DPL packed bytes at byte offset `off` (kDrop: no store).
-template <int DPL>
+// Store DPL packed bytes at byte offset `off` (kDrop: no store).  NT: the
+// stores are non-temporal (cache policy NT).  The path kernels use it from 8
+// disparities per lane up: those volumes (4 x N x D bytes, about 1 GB at 1080p
+// D=128) are written once, read once by the argmin and outgrow the last-level
+// cache, so they bypass it and leave it to the census codes every direction
+// re-reads (plain 1080p D=128 0.715 -> 0.657 ms).  Narrower lanes store 4
+// bytes or fewer per lane and step, which NT writes reach DRAM as partial
+// lines (21 % more write traffic at D=64, for no gain in time); the windowed
+// path's smaller volumes stay cached for its argmin.
+constexpr int kStoreNT = 2; // buffer-store cache policy: non-temporal
+template <int DPL, bool NT = true>
 __device__ __forceinline__ void st_bytes(rsrc_t r, uint32_t off, const uint32_t (&v)[DPL])
 {
+    constexpr int pol = NT ? kStoreNT : 0;
     if constexpr (DPL == 1) {
-        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v[0], r, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b8((uint8_t)v[0], r, off, 0, pol);
     } else if constexpr (DPL == 2) {
-        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(v[0] | (v[1] << 8)), r, off, 0, 0);
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(v[0] | (v[1] << 8)), r, off, 0, pol);
     } else {
         uint32_t w[DPL / 4];
 #pragma unroll
         for (int k = 0; k < DPL / 4; k++)
             w[k] = pack4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
         if constexpr (DPL == 4) {
-            __builtin_amdgcn_raw_buffer_store_b32(w[0], r, off, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(w[0], r, off, 0, pol);
         } else if constexpr (DPL == 8) {
             typedef uint32_t v2u __attribute__((ext_vector_type(2)));
             const v2u q = {w[0], w[1]};
-            __builtin_amdgcn_raw_buffer_store_b64(q, r, off, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b64(q, r, off, 0, pol);
         } else {
 #pragma unroll
             for (int k = 0; k < DPL / 4; k += 4) {
                 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
                 const v4u q = {w[k], w[k + 1], w[k + 2], w[k + 3]};
-                __builtin_amdgcn_raw_buffer_store_b128(q, r, off == kDrop ? kDrop : off + 4 * k, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(q, r, off == kDrop ? kDrop : off + 4 * k, 0, pol);
             }
         }
     }
@@ -402,7 +412,7 @@ __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, Li
             pl[k] = ld64(rl, (uint32_t)(row + xp) * 8u);
             pn[k] = ld64(rr, (uint32_t)(row + xp + enter) * 8u + nbias);
             sgm_step<DPL, SAT>(L, C, mn, P1, P2);
-            st_bytes<DPL>(ro, (!CHECK || s < steps) ? off : kDrop, L);
+            st_bytes<DPL, (DPL >= 8)>(ro, (!CHECK || s < steps) ? off : kDrop, L);
             off += ostep;
             x += DX;
             // keep each step's uses of the ring in program order, so a
@@ -542,7 +552,7 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
             }
             if constexpr (CHECK)
                 off = s < steps ? off : kDrop;
-            st_bytes<DPL>(ro, off | dead, L);
+            st_bytes<DPL, (DPL >= 8)>(ro, off | dead, L);
             pix += pstep;
             x += dx;
             y += dy;
@@ -961,7 +971,7 @@ __device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask
                     }
                 }
                 const uint32_t off = real ? sb_f + (uint32_t)(D * sstep(i)) : kDrop;
-                st_bytes<NB>(ro, off, S.SF);
+                st_bytes<NB, false>(ro, off, S.SF);
                 if (CHECK && q < q0) {
 #pragma unroll
                     for (int j = 0; j < NB; j++)
@@ -1130,7 +1140,7 @@ __device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask
                             S.SF[j] = umin(S.SF[j], kWinAbsent);
                     }
                 }
-                st_bytes<NB>(ro, real ? so_f : kDrop, S.SF);
+                st_bytes<NB, false>(ro, real ? so_f : kDrop, S.SF);
                 if (CHECK && q < q0) {
 #pragma unroll
                     for (int j = 0; j < NB; j++)
