@@ -191,8 +191,9 @@ __device__ __forceinline__ uint64_t ld64(rsrc_t r, uint32_t off)
 // cache, so they bypass it and leave it to the census codes every direction
 // re-reads (plain 1080p D=128 0.715 -> 0.657 ms).  Narrower lanes store 4
 // bytes or fewer per lane and step, which NT writes reach DRAM as partial
-// lines (21 % more write traffic at D=64, for no gain in time); the windowed
-// path's smaller volumes stay cached for its argmin.
+// lines (21 % more write traffic at D=64, for no gain in time).  The windowed
+// path stores its full-layout tiles NT at D=128 too, so that its small dense
+// narrow volumes stay cached for the argmin.
 constexpr int kStoreNT = 2; // buffer-store cache policy: non-temporal
 template <int DPL, bool NT = true>
 __device__ __forceinline__ void st_bytes(rsrc_t r, uint32_t off, const uint32_t (&v)[DPL])
@@ -971,7 +972,7 @@ __device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask
                     }
                 }
                 const uint32_t off = real ? sb_f + (uint32_t)(D * sstep(i)) : kDrop;
-                st_bytes<NB, false>(ro, off, S.SF);
+                st_bytes<NB, (NB >= 8)>(ro, off, S.SF);
                 if (CHECK && q < q0) {
 #pragma unroll
                     for (int j = 0; j < NB; j++)
@@ -1140,7 +1141,7 @@ __device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask
                             S.SF[j] = umin(S.SF[j], kWinAbsent);
                     }
                 }
-                st_bytes<NB, false>(ro, real ? so_f : kDrop, S.SF);
+                st_bytes<NB, (NB >= 8)>(ro, real ? so_f : kDrop, S.SF);
                 if (CHECK && q < q0) {
 #pragma unroll
                     for (int j = 0; j < NB; j++)
