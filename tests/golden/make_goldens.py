@@ -117,6 +117,15 @@ def main():
             ("ref_640x480_d128", ref_scene(640, 480, 128, 1), "plain", 128, 4),
             ("ref_1280x720_d64", ref_scene(1280, 720, 64, 1), "plain", 64, 4),
             ("ref_960x540_d128", ref_scene(960, 540, 128, 1), "plain", 128, 4),
+            # D sweep over resolutions (the reference generator at each D): with the
+            # cells above and the on-disk goldens, every D of 32..256 at 640x480,
+            # 1280x720 and 1920x1080
+            ("grid_640x480_d32", ref_scene(640, 480, 32, 1), "plain", 32, 4),
+            ("grid_640x480_d256", ref_scene(640, 480, 256, 1), "plain", 256, 4),
+            ("grid_1280x720_d32", ref_scene(1280, 720, 32, 1), "plain", 32, 4),
+            ("grid_1280x720_d128", ref_scene(1280, 720, 128, 1), "plain", 128, 4),
+            ("grid_1280x720_d256", ref_scene(1280, 720, 256, 1), "plain", 256, 4),
+            ("grid_1920x1080_d32", ref_scene(1920, 1080, 32, 1), "plain", 32, 4),
             # 8 paths (SGM_PATHS=8 in the reference oracle)
             ("p8_640x480_d64", planes(640, 480, 63, 12), "plain", 64, 8),
             ("p8_ref_640x480_d64", ref_scene(640, 480, 64, 1), "plain", 64, 8),
