@@ -40,4 +40,10 @@ static int hip_run(const uint8_t *left, const uint8_t *right, int W, int H, uint
     return 0;
 }
 
+/* The 8-path build names itself apart: the cost gate keys its calibration by
+ * implementation, board, resolution and D, not by the path count. */
+#if SGM_PATHS == 8
+const sgm_impl SGM_IMPL = {"hip_p8", hip_run};
+#else
 const sgm_impl SGM_IMPL = {"hip", hip_run};
+#endif
