@@ -629,12 +629,14 @@ __device__ __forceinline__ uint32_t group_min(uint32_t v)
     return v;
 }
 
+constexpr int kWtaBatch = 4; // pixel batches per wta_batch call: 4 x 4 directions x 16 B in flight per lane
+
 // One batch of U*PPW pixels starting at b0 (every lane of the wave calls it).
 template <int LPP, int NDIRS>
 __device__ __forceinline__ void wta_batch(const uint8_t *__restrict__ L, size_t dir_stride, int npix,
                                           uint8_t *__restrict__ disp, int b0)
 {
-    constexpr int D = 16 * LPP, PPW = 64 / LPP, U = 2;
+    constexpr int D = 16 * LPP, PPW = 64 / LPP, U = kWtaBatch;
     const int lane = threadIdx.x & 63, sub = lane % LPP, slot = lane / LPP;
     uint4 v[U][NDIRS];
     int pix[U];
@@ -679,7 +681,6 @@ __device__ __forceinline__ void wta_batch(const uint8_t *__restrict__ L, size_t 
     }
 }
 
-constexpr int kWtaBatch = 2; // pixel batches per wta_batch call
 
 // ---------------------------------------------------------------------------
 // windowed SGM (build-defined; no reference counterpart)

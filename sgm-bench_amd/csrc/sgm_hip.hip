@@ -245,7 +245,7 @@ static int launch_wta(hipStream_t s, const uint8_t *vol, size_t dir_stride, int 
 {
     REQUIRE(ndirs >= 1 && ndirs <= 8 && npix < (1ull << 31), "sgmh: bad argmin shape");
     const int ppw = 64 / (D / 16);                    // pixels per wave per batch
-    const size_t waves = (npix + (size_t)ppw * 2 - 1) / ((size_t)ppw * 2);
+    const size_t waves = (npix + (size_t)ppw * kWtaBatch - 1) / ((size_t)ppw * kWtaBatch);
     const int blocks = (int)std::min<size_t>((waves + 3) / 4, 8192);
     const int n = (int)npix;
     REQUIRE(ndirs == 4 || ndirs == 8, "sgmh: argmin over %d directions", ndirs);
