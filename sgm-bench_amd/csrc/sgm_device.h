@@ -46,11 +46,24 @@ __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uin
                                             uint32_t *__restrict__ tile)
 {
     uint8_t *t8 = reinterpret_cast<uint8_t *>(tile);
-    __syncthreads(); // the previous tile of a persistent loop is no longer read
-    for (int i = threadIdx.x; i < kCenRows * kCenRowBytes; i += 256) {
+    // every staging load is issued before the first LDS write, so a thread
+    // waits for one memory latency rather than one per byte
+    constexpr int kBytes = kCenRows * kCenRowBytes, kIters = (kBytes + 255) / 256;
+    uint32_t px[kIters];
+#pragma unroll
+    for (int k = 0; k < kIters; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
         const int r = i / kCenRowBytes, c = i - r * kCenRowBytes;
         const int yy = clampi(y0 - 3 + r, 0, H - 1), xx = clampi(x0 - 4 + c, 0, W - 1);
-        t8[i] = img[(size_t)yy * W + xx];
+        if (k + 1 < kIters || i < kBytes)
+            px[k] = img[(size_t)yy * W + xx];
+    }
+    __syncthreads(); // the previous tile of a persistent loop is no longer read
+#pragma unroll
+    for (int k = 0; k < kIters; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        if (k + 1 < kIters || i < kBytes)
+            t8[i] = (uint8_t)px[k];
     }
     __syncthreads();
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -336,7 +349,13 @@ constexpr uint32_t kFar = 0x80000000u;
 // multiple of DPL), so the shift itself moves nothing.
 // The path starts from Lp = 0, mn = 0: with P1 > 0 that step yields L = C
 // exactly, the reference's first-pixel rule (sgm_ref.c:90-91), branch-free.
-template <int DPL, bool SAT, int DX, class A>
+// BCAST: lane 0 of each row loads the left code and the XORs take it through
+// a row_newbcast DPP operand.  Same instruction count, but it changes how the
+// row steps issue beside the column waves that share their SIMDs: measured
+// 3-4 % faster path kernels at D = 64 where there is at least one row wave
+// per CU (1280x720, 1920x1080), 6 % slower where rows are few (640x480) and
+// 1 % slower at D = 128, so the launch picks it by frame height and D.
+template <int DPL, bool SAT, int DX, bool BCAST, class A>
 __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, LineGeom ln,
                                                 int l, int maxsteps)
 {
@@ -367,10 +386,11 @@ __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, Li
     // lane that takes the entering code loads it, the others read 0
     const int enter = DX > 0 ? 0 : -(D - 1);
     const uint32_t nbias = l == (DX > 0 ? 0 : kRowLanes - 1) ? 0u : kFar;
+    const uint32_t lbias = BCAST && l != 0 ? kFar : 0u; // BCAST: lane 0 alone loads the left code
     uint64_t pl[PF], pn[PF];
 #pragma unroll
     for (int k = 0; k < PF; k++) {
-        pl[k] = ld64(rl, (uint32_t)(row + x + k * DX) * 8u);
+        pl[k] = ld64(rl, (uint32_t)(row + x + k * DX) * 8u + lbias);
         pn[k] = ld64(rr, (uint32_t)(row + x + k * DX + enter) * 8u + nbias);
         __builtin_amdgcn_sched_barrier(0); // slot 0 first: its wait is the shortest
     }
@@ -398,7 +418,7 @@ __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, Li
                 const int slot = ((k - 1) % DPL + DPL) % DPL;     // logical DPL-1 at step s
                 phys[slot] = dpp64_shl1(pn[k], phys[slot]);      // was logical 0
             }
-            const uint64_t cL = pl[k];
+            const uint64_t cL = BCAST ? row_bcast0(pl[k]) : pl[k];
             uint32_t C[DPL];
             const int rel = x - lbase;
 #pragma unroll
@@ -410,7 +430,7 @@ __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, Li
             // ring slot k is dead now: refill it for step s + PF in place
             __builtin_amdgcn_sched_barrier(0);
             const int xp = x + PF * DX;
-            pl[k] = ld64(rl, (uint32_t)(row + xp) * 8u);
+            pl[k] = ld64(rl, (uint32_t)(row + xp) * 8u + lbias);
             pn[k] = ld64(rr, (uint32_t)(row + xp + enter) * 8u + nbias);
             sgm_step<DPL, SAT>(L, C, mn, P1, P2);
             st_bytes<DPL, (DPL >= 8)>(ro, (!CHECK || s < steps) ? off : kDrop, L);
@@ -573,7 +593,7 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
 // Scanlines grp*4 .. grp*4+3 of direction t, one per 16-lane row of the wave.
 // A is any struct with PathArgs' frame fields (cl, cr, cost, W, H, D, P2,
 // invalid, out_bytes).
-template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, class A>
+template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, class A, bool BCAST = false>
 __device__ __forceinline__ void path_group(const A &a, const DirTask &t, int grp)
 {
     const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
@@ -590,9 +610,9 @@ __device__ __forceinline__ void path_group(const A &a, const DirTask &t, int grp
     if constexpr (FROM_CENSUS && !DECIM) {
         if (t.dy == 0) {
             if (t.dx > 0)
-                path_horizontal<DPL, SAT, 1, A>(a, t, ln, l, maxsteps);
+                path_horizontal<DPL, SAT, 1, BCAST, A>(a, t, ln, l, maxsteps);
             else
-                path_horizontal<DPL, SAT, -1, A>(a, t, ln, l, maxsteps);
+                path_horizontal<DPL, SAT, -1, BCAST, A>(a, t, ln, l, maxsteps);
             return;
         }
     }

@@ -89,8 +89,9 @@ __global__ __launch_bounds__(256) void k_cost(const uint64_t *__restrict__ cl,
 // -- share an L2.  Rows gain nothing from it, and diagonals must stay
 // interleaved (their lengths vary along the range), so they keep block order.
 constexpr int kXcds = 8;
+constexpr int kRowBcastMinH = 512; // 2 x 512 / 4 row waves = 256, one per CU
 
-template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM>
+template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, bool BCAST = false>
 __global__ __launch_bounds__(64) void k_path(PathArgs a)
 {
     int k = 0;
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(64) void k_path(PathArgs a)
     const int grp = t.dx == 0 ? (i % kXcds) * (n8 / kXcds) + i / kXcds : i;
     if (grp * kLinesPerWave >= t.nlines)
         return; // padding block
-    path_group<DPL, FROM_CENSUS, SAT, DECIM, PathArgs>(a, t, grp);
+    path_group<DPL, FROM_CENSUS, SAT, DECIM, PathArgs, BCAST>(a, t, grp);
 }
 
 template <int LPP, int NDIRS>
@@ -173,15 +174,15 @@ static int launch_census(hipStream_t s, const uint8_t *img0, uint64_t *out0, int
     return 0;
 }
 
-template <bool FROM_CENSUS, bool SAT, bool DECIM>
+template <bool FROM_CENSUS, bool SAT, bool DECIM, bool BCAST = false>
 static int launch_path_t(hipStream_t s, const PathArgs &a, int blocks)
 {
     switch (a.D / 16) {
-    case 1: hipLaunchKernelGGL((k_path<1, FROM_CENSUS, SAT, DECIM>), dim3(blocks), dim3(64), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_path<2, FROM_CENSUS, SAT, DECIM>), dim3(blocks), dim3(64), 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_path<4, FROM_CENSUS, SAT, DECIM>), dim3(blocks), dim3(64), 0, s, a); break;
-    case 8: hipLaunchKernelGGL((k_path<8, FROM_CENSUS, SAT, DECIM>), dim3(blocks), dim3(64), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((k_path<16, FROM_CENSUS, SAT, DECIM>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_path<1, FROM_CENSUS, SAT, DECIM, BCAST>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_path<2, FROM_CENSUS, SAT, DECIM, BCAST>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_path<4, FROM_CENSUS, SAT, DECIM, BCAST>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 8: hipLaunchKernelGGL((k_path<8, FROM_CENSUS, SAT, DECIM, BCAST>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_path<16, FROM_CENSUS, SAT, DECIM, BCAST>), dim3(blocks), dim3(64), 0, s, a); break;
     default: set_err("sgmh: bad D %d", a.D); return -1;
     }
     CK(hipGetLastError());
@@ -220,8 +221,11 @@ static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndi
         const int groups = (t.nlines + kLinesPerWave - 1) / kLinesPerWave;
         blocks += (groups + kXcds - 1) / kXcds * kXcds;
     }
+    // the broadcast row variant pays where row waves are at least one per CU,
+    // at D = 64 (measured; at D = 128 it is 1 % slower)
     if (from_census && !sat && !decimate)
-        return launch_path_t<true, false, false>(s, a, blocks);
+        return a.H >= kRowBcastMinH && a.D == 64 ? launch_path_t<true, false, false, true>(s, a, blocks)
+                                                 : launch_path_t<true, false, false>(s, a, blocks);
     if (!from_census && sat && !decimate)
         return launch_path_t<false, true, false>(s, a, blocks);
     if (!from_census && sat && decimate)

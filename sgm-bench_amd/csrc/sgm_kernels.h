@@ -28,6 +28,14 @@ __device__ __forceinline__ uint32_t dpp(uint32_t v)
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
 }
 
+constexpr int kRowBcast0 = 0x150; // row_newbcast:0, lane 0 of each row to the whole row
+
+// The value lane 0 of each 16-lane row holds, in every lane of the row.
+__device__ __forceinline__ uint64_t row_bcast0(uint64_t v)
+{
+    return ((uint64_t)dpp<kRowBcast0>((uint32_t)(v >> 32)) << 32) | dpp<kRowBcast0>((uint32_t)v);
+}
+
 // DPP move where lanes without a source keep `old`.
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_or(uint32_t old, uint32_t v)

@@ -56,7 +56,9 @@ def test_path_volumes(ctx, D, H, W):
 
 @pytest.mark.parametrize("H,W,D,paths", [
     (16, 16, 16, 4), (17, 33, 32, 4), (61, 97, 64, 4), (61, 97, 64, 8), (300, 40, 64, 4),
-    (120, 200, 128, 4), (120, 200, 128, 8), (20, 1000, 256, 4), (64, 64, 256, 8)])
+    (120, 200, 128, 4), (120, 200, 128, 8), (20, 1000, 256, 4), (64, 64, 256, 8),
+    # D = 64 frames of 512 rows or more take the broadcast row variant
+    (515, 77, 64, 4), (513, 130, 64, 8)])
 def test_plain_matches_oracle(ctx, H, W, D, paths):
     L, R, _ = sgm_amd.scene_planes(W, H, min(D - 1, W - 1), W * H + D, planes=(3, 6))
     got = ctx.plain(L, R, D, paths)
