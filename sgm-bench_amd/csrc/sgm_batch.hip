@@ -129,7 +129,7 @@ __global__ __launch_bounds__(64) void k_batch_reset(int *counter) { if (threadId
 
 __global__ __launch_bounds__(256) void k_batch_pre(const PreTask *__restrict__ tasks, int ntasks, int nitems)
 {
-    __shared__ uint32_t tile[kCenRows * kCenRowWords];
+    __shared__ __align__(16) uint32_t tile[kCenRows * kCenRowWords];
     int begin, end;
     chunk_of(blockIdx.x, gridDim.x, nitems, begin, end);
     if (begin >= end)

@@ -33,19 +33,51 @@ static const int kDirsMs[8][3] = {{1, 0, 40}, {-1, 0, 40}, {0, 1, 20}, {0, -1, 2
 constexpr int kCenTX = 64, kCenTY = 16;
 constexpr int kCenRowBytes = kCenTX + 8;         // 72: x0-4 .. x0+67
 constexpr int kCenRows = kCenTY + 6;             // 22: y0-3 .. y0+18
-constexpr int kCenRowWords = kCenRowBytes / 4;   // 18
+constexpr int kCenRowWords = kCenRowBytes / 2;   // 36: the tile holds each pixel as a uint16
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+typedef uint16_t cen_u16x2 __attribute__((ext_vector_type(2)));
+
+// (a - b) on the two uint16 halves independently (v_pk_sub_u16)
+__device__ __forceinline__ uint32_t pk_sub_u16(uint32_t a, uint32_t b)
+{
+    cen_u16x2 x, y;
+    __builtin_memcpy(&x, &a, 4);
+    __builtin_memcpy(&y, &b, 4);
+    const cen_u16x2 d = x - y;
+    uint32_t r;
+    __builtin_memcpy(&r, &d, 4);
+    return r;
+}
+
+// (a & m) | b in one v_and_or_b32: written out, the compiler re-associates a
+// chain of them into ands and three-input ors, which costs a third more
+__device__ __forceinline__ uint32_t and_or(uint32_t a, uint32_t m, uint32_t b)
+{
+    uint32_t r;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(m), "v"(b));
+    return r;
+}
+
 // One 64x16 output tile by 256 threads (all of them must call it): the
 // (64+8)x(16+6) input window is staged in `tile` (kCenRows*kCenRowWords words
-// of LDS) with clamped coordinates, then each thread builds the codes of 4
-// consecutive pixels from 7 rows x 3 dwords of LDS.
+// of LDS, 8-byte aligned, one uint16 per pixel) with clamped coordinates, then
+// each thread builds the codes of 4 consecutive pixels from 7 rows x 6 dwords
+// of LDS.
+//
+// Two pixels are compared at once: with a neighbour pair n and the centre
+// pair c as uint16 halves, (n - c) per half is n - c < 256 when n >= c and
+// 0xFF01..0xFFFF when n < c, so bits 8..15 of each half are eight copies of
+// the census bit (nb < centre, common/sgm_params.h:45-47).  Bit n of a code
+// is taken into bit 8 + n % 8 of word n / 8 by one v_and_or_b32, so word k
+// ends up holding code byte k of the pair's first pixel in its byte 1 and of
+// the second pixel in its byte 3; byte permutes then assemble the codes.
 __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uint64_t *__restrict__ out,
                                             int W, int H, int ostride, int x0, int y0,
                                             uint32_t *__restrict__ tile)
 {
-    uint8_t *t8 = reinterpret_cast<uint8_t *>(tile);
+    uint16_t *t16 = reinterpret_cast<uint16_t *>(tile);
     // every staging load is issued before the first LDS write, so a thread
     // waits for one memory latency rather than one per byte
     constexpr int kBytes = kCenRows * kCenRowBytes, kIters = (kBytes + 255) / 256;
@@ -63,46 +95,60 @@ __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uin
     for (int k = 0; k < kIters; k++) {
         const int i = (int)threadIdx.x + 256 * k;
         if (k + 1 < kIters || i < kBytes)
-            t8[i] = (uint8_t)px[k];
+            t16[i] = (uint16_t)px[k];
     }
     __syncthreads();
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int x = x0 + 4 * tx, y = y0 + ty;
     if (y >= H || x >= W)
         return;
-    // bytes 4tx .. 4tx+11 of the 7 tile rows around this thread's row
-    uint32_t win[7][3];
+    // pixels x-4 .. x+7 of the 7 tile rows around this thread's row, as
+    // uint16 pairs: win[r][k] holds pixels x-4+2k (low half) and x-3+2k
+    uint32_t win[7][6];
+    const uint2 *t2 = reinterpret_cast<const uint2 *>(tile);
 #pragma unroll
     for (int r = 0; r < 7; r++)
 #pragma unroll
-        for (int k = 0; k < 3; k++)
-            win[r][k] = tile[(ty + r) * kCenRowWords + tx + k];
+        for (int k = 0; k < 3; k++) {
+            const uint2 v = t2[(ty + r) * (kCenRowWords / 2) + tx + k];
+            win[r][2 * k] = v.x;
+            win[r][2 * k + 1] = v.y;
+        }
+    // the pair of pixels x-4+o, x-3+o of row r
+    auto pair = [&](int r, int o) -> uint32_t {
+        return (o & 1) ? __builtin_amdgcn_alignbit(win[r][(o + 1) >> 1], win[r][(o - 1) >> 1], 16)
+                       : win[r][o >> 1];
+    };
     uint64_t code[4];
 #pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const uint32_t centre = (win[3][(p + 4) >> 2] >> (8 * ((p + 4) & 3))) & 0xFFu;
-        uint32_t lo = 0, hi = 0;
-        // bit n: n-th neighbour in (dy outer -3..3, dx inner -4..4) order,
-        // centre skipped (common/sgm_params.h:45-47).  The neighbours are
-        // taken in reverse and shifted in at the bottom, so bit n lands at
-        // position n: nb - centre has bit 31 set iff nb < centre, and
-        // v_alignbit(word, d, 31) = word << 1 | d >> 31.
-        int n = 61;
+    for (int j = 0; j < 4; j += 2) {
+        const uint32_t centre = pair(3, j + 4);
+        uint32_t wd[8];
+        int n = 0; // bit n: n-th neighbour in (dy outer -3..3, dx inner -4..4) order, centre skipped
 #pragma unroll
-        for (int r = 6; r >= 0; r--)
+        for (int r = 0; r < 7; r++)
 #pragma unroll
-            for (int c = 8; c >= 0; c--) {
+            for (int c = 0; c < 9; c++) {
                 if (r == 3 && c == 4)
                     continue;
-                const int b = p + c; // byte index of (x+p + c-4) in the window
-                const uint32_t d = ((win[r][b >> 2] >> (8 * (b & 3))) & 0xFFu) - centre;
-                if (n >= 32)
-                    hi = __builtin_amdgcn_alignbit(hi, d, 31);
+                const uint32_t f = pk_sub_u16(pair(r, j + c), centre);
+                const int k = n >> 3, m = n & 7;
+                const uint32_t mask = 0x01000100u << m;
+                if (m == 0)
+                    wd[k] = f & mask;
                 else
-                    lo = __builtin_amdgcn_alignbit(lo, d, 31);
-                n--;
+                    wd[k] = and_or(f, mask, wd[k]);
+                n++;
             }
-        code[p] = ((uint64_t)hi << 32) | lo;
+        // byte 1 of each word -> the first pixel's code, byte 3 -> the second's
+        const uint32_t t01 = __builtin_amdgcn_perm(wd[1], wd[0], 0x07030501u);
+        const uint32_t t23 = __builtin_amdgcn_perm(wd[3], wd[2], 0x07030501u);
+        const uint32_t t45 = __builtin_amdgcn_perm(wd[5], wd[4], 0x07030501u);
+        const uint32_t t67 = __builtin_amdgcn_perm(wd[7], wd[6], 0x07030501u);
+        code[j] = ((uint64_t)__builtin_amdgcn_perm(t67, t45, 0x05040100u) << 32) |
+                  __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+        code[j + 1] = ((uint64_t)__builtin_amdgcn_perm(t67, t45, 0x07060302u) << 32) |
+                      __builtin_amdgcn_perm(t23, t01, 0x07060302u);
     }
     uint64_t *o = out + (size_t)y * ostride + x;
     if (x + 3 < W) {

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void k_census(const uint8_t *__restrict__ img0
                                                 uint64_t *__restrict__ out0, uint64_t *__restrict__ out1, int W,
                                                 int H, int ostride)
 {
-    __shared__ uint32_t tile[kCenRows * kCenRowWords];
+    __shared__ __align__(16) uint32_t tile[kCenRows * kCenRowWords];
     const bool second = blockIdx.z != 0;
     census_tile(second ? img1 : img0, second ? out1 : out0, W, H, ostride, blockIdx.x * kCenTX,
                 blockIdx.y * kCenTY, tile);
@@ -189,10 +189,11 @@ static int launch_path_t(hipStream_t s, const PathArgs &a, int blocks)
     return 0;
 }
 
-// Fill the direction table (longest lines first, so the first blocks the
-// dispatcher places are the longest-running ones) and launch.  Two variants
-// are instantiated: census-fed plain paths and cost-volume-fed saturating
-// multiscale paths (full or decimated store).
+// Fill the direction table and launch: census-fed plain paths with the row
+// directions last (see below), the others with the longest lines first, so
+// the first blocks the dispatcher places are the longest-running ones.  Two
+// variants are instantiated: census-fed plain paths and cost-volume-fed
+// saturating multiscale paths (full or decimated store).
 static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndirs,
                         bool from_census, bool sat, uint8_t *vol, size_t dir_stride, int decimate)
 {
@@ -205,7 +206,16 @@ static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndi
     for (int k = 0; k < ndirs; k++)
         order[k] = k;
     auto len = [&](int k) { return dirs[k][1] == 0 ? a.W : (dirs[k][0] == 0 ? a.H : std::min(a.W, a.H)); };
-    std::stable_sort(order, order + ndirs, [&](int p, int q) { return len(p) > len(q); });
+    // Census-fed plain paths put the row directions last.  The hardware places
+    // single-wave workgroups on SIMDs by wave count, not by role: with the rows
+    // first it put two row waves on one SIMD in some frames (3-30 % of them,
+    // depending on the kernel that ran before), and such a frame's paths take
+    // 0.345 instead of 0.26 ms at 1080p.  Placed after the columns, the rows
+    // never shared a SIMD in the frames measured, at the same median time.
+    if (from_census && !sat && !decimate)
+        std::stable_sort(order, order + ndirs, [&](int p, int q) { return (dirs[p][1] == 0) < (dirs[q][1] == 0); });
+    else
+        std::stable_sort(order, order + ndirs, [&](int p, int q) { return len(p) > len(q); });
     int blocks = 0;
     a.ndirs = ndirs;
     for (int i = 0; i < ndirs; i++) {
