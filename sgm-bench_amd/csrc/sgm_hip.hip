@@ -408,10 +408,13 @@ int window_rest(sgmh_ctx *ctx, hipStream_t s, int W, int H, int D, const uint8_t
     a.out_bytes = (uint32_t)(N * D);
     a.ndirs = 4;
     int blocks = 0;
+    // columns first, rows last, as for the plain paths (measured: 1 % faster
+    // here); the argmin sums the four volumes, so their order is free
     for (int k = 0; k < 4; k++) {
+        const int o = (k + 2) & 3;
         DirTask &t = a.dir[k];
-        t.dx = kDirsPlain[k][0];
-        t.dy = kDirsPlain[k][1];
+        t.dx = kDirsPlain[o][0];
+        t.dy = kDirsPlain[o][1];
         t.P1 = (int)kPlainP1;
         t.nlines = t.dy == 0 ? H : W;
         t.first_block = blocks;
