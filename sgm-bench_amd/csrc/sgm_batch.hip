@@ -168,13 +168,14 @@ __global__ __launch_bounds__(256) void k_batch_cost(const CostTask *__restrict__
     for (int item = begin; item < end; item++) {
         c.advance(item);
         const CostTask &tk = c.cur;
-        const size_t p = (size_t)(item - tk.first_item) * 16 + (threadIdx.x >> 6) * 4 + g;
-        if (p >= (size_t)tk.W * tk.H)
+        const uint32_t p = (uint32_t)(item - tk.first_item) * 16u + (threadIdx.x >> 6) * 4u + (uint32_t)g;
+        if (p >= (uint32_t)(tk.W * tk.H))
             continue;
+        const int y = (int)(p / (uint32_t)tk.W), x = (int)(p - (uint32_t)y * (uint32_t)tk.W);
         switch (tk.D) {
-        case 32: cost_px<2>(tk.cl, tk.cr, tk.W, tk.D, tk.invalid, tk.merge, tk.C, p, l); break;
-        case 64: cost_px<4>(tk.cl, tk.cr, tk.W, tk.D, tk.invalid, tk.merge, tk.C, p, l); break;
-        default: cost_px<8>(tk.cl, tk.cr, tk.W, tk.D, tk.invalid, tk.merge, tk.C, p, l); break;
+        case 32: cost_px<2>(tk.cl, tk.cr, tk.W, tk.H, tk.D, tk.invalid, tk.merge, tk.C, x, y, l); break;
+        case 64: cost_px<4>(tk.cl, tk.cr, tk.W, tk.H, tk.D, tk.invalid, tk.merge, tk.C, x, y, l); break;
+        default: cost_px<8>(tk.cl, tk.cr, tk.W, tk.H, tk.D, tk.invalid, tk.merge, tk.C, x, y, l); break;
         }
     }
 }

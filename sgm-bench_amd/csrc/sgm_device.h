@@ -188,34 +188,6 @@ __device__ __forceinline__ void sobel_px(const uint8_t *__restrict__ img, int W,
     out[(size_t)y * W + x] = (uint8_t)clampi(128 + (gx >> 3), 0, 255);
 }
 
-// Materialised cost volume (multiscale): one pixel per 16-lane row.
-//   merge == nullptr: C = popcount(cl ^ cr[x-d]) or `invalid`      (sgm_ms_ref.c:66-72)
-//   merge != nullptr: C = (C + merge[(y/2)*hw + x/2][d/2] + 1) >> 1 (sgm_ms_ref.c:184-188)
-template <int DPL>
-__device__ __forceinline__ void cost_px(const uint64_t *__restrict__ cl, const uint64_t *__restrict__ cr,
-                                        int W, int D, uint32_t invalid, const uint8_t *__restrict__ merge,
-                                        uint8_t *__restrict__ C, size_t p, int l)
-{
-    const int y = (int)(p / W), x = (int)(p - (size_t)y * W);
-    const uint64_t a = cl[p];
-    const uint64_t *crow = cr + (size_t)y * W;
-    uint32_t c[DPL];
-#pragma unroll
-    for (int j = 0; j < DPL; j++) {
-        const int xr = x - (l * DPL + j);
-        const uint64_t b = crow[xr < 0 ? 0 : xr];
-        c[j] = xr < 0 ? invalid : (uint32_t)__popcll(a ^ b);
-    }
-    if (merge) {
-        const int hw = W / 2;
-        const uint8_t *m = merge + ((size_t)(y / 2) * hw + x / 2) * D;
-#pragma unroll
-        for (int j = 0; j < DPL; j++)
-            c[j] = (c[j] + m[(l * DPL + j) >> 1] + 1) >> 1;
-    }
-    store_bytes<DPL>(C + p * D + l * DPL, c);
-}
-
 // ---------------------------------------------------------------------------
 // path aggregation: all directions of a frame in one launch
 // ---------------------------------------------------------------------------
@@ -241,6 +213,70 @@ __device__ __forceinline__ uint64_t ld64(rsrc_t r, uint32_t off)
 {
     const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     return ((uint64_t)v[1] << 32) | v[0];
+}
+
+// Materialised cost volume (multiscale): pixel (x, y), one pixel per 16-lane
+// row, lane l owning disparities l*DPL .. l*DPL+DPL-1.
+//   merge == nullptr: C = popcount(cl ^ cr[x-d]) or `invalid`      (sgm_ms_ref.c:66-72)
+//   merge != nullptr: C = (C + merge[(y/2)*hw + x/2][d/2] + 1) >> 1 (sgm_ms_ref.c:184-188)
+// The lane's right codes cr[x-d] are contiguous and descending in d: they
+// come 16 bytes at a time through a buffer resource (on rows below the first,
+// codes left of x = 0 are the previous row's and are replaced by `invalid`).
+// On row 0 a window starting left of x = 0 is read code by code with clamped
+// indices: a 16-byte load straddling offset 0 would be out of range as a
+// whole.  The merge terms of a lane are DPL/2 consecutive bytes, one load.
+template <int DPL>
+__device__ __forceinline__ void cost_px(const uint64_t *__restrict__ cl, const uint64_t *__restrict__ cr,
+                                        int W, int H, int D, uint32_t invalid, const uint8_t *__restrict__ merge,
+                                        uint8_t *__restrict__ C, int x, int y, int l)
+{
+    const uint32_t p = (uint32_t)(y * W + x);
+    const uint64_t a = cl[p];
+    const rsrc_t rr = make_rsrc(cr, (uint32_t)(W * H) * 8u);
+    const int lo = x - l * DPL - (DPL - 1); // lowest code of the window, relative to the row start
+    uint64_t b[DPL];                        // b[j] = cr[row + x - l*DPL - j]
+    const uint32_t base = (uint32_t)(y * W + lo) * 8u;
+    if (DPL >= 2 && (y > 0 || lo >= 0)) {
+#pragma unroll
+        for (int i = 0; i < DPL; i += 2) {
+            const auto q = __builtin_amdgcn_raw_buffer_load_b128(rr, base + 8u * i, 0, 0);
+            b[DPL - 1 - i] = ((uint64_t)q[1] << 32) | q[0];
+            b[DPL - 2 - i] = ((uint64_t)q[3] << 32) | q[2];
+        }
+    } else {
+        // row 0, window starting left of x = 0: clamped in-range loads (codes
+        // left of x = 0 are replaced by `invalid` below)
+        const uint64_t *crow = cr + (size_t)y * W;
+#pragma unroll
+        for (int j = 0; j < DPL; j++) {
+            const int xr = x - (l * DPL + j);
+            b[j] = crow[xr < 0 ? 0 : xr];
+        }
+    }
+    uint32_t c[DPL];
+#pragma unroll
+    for (int j = 0; j < DPL; j++) {
+        const uint32_t h = (uint32_t)__popcll(a ^ b[j]);
+        c[j] = x - (l * DPL + j) < 0 ? invalid : h;
+    }
+    if (merge) {
+        const uint8_t *m = merge + ((size_t)(y / 2) * (W / 2) + x / 2) * D;
+        uint64_t mw;
+        if constexpr (DPL == 1)
+            mw = m[l >> 1];
+        else if constexpr (DPL == 2)
+            mw = m[l];
+        else if constexpr (DPL == 4)
+            mw = *reinterpret_cast<const uint16_t *>(m + 2 * l);
+        else if constexpr (DPL == 8)
+            mw = *reinterpret_cast<const uint32_t *>(m + 4 * l);
+        else
+            mw = *reinterpret_cast<const uint64_t *>(m + 8 * l);
+#pragma unroll
+        for (int j = 0; j < DPL; j++)
+            c[j] = (c[j] + (uint32_t)((mw >> (8 * (j >> 1))) & 0xFFu) + 1) >> 1;
+    }
+    store_bytes<DPL>(C + (size_t)p * D + l * DPL, c);
 }
 
 // Store DPL packed bytes at byte offset `off` (kDrop: no store).  NT: the

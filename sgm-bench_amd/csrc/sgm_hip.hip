@@ -66,7 +66,8 @@ __global__ __launch_bounds__(256) void k_sobel(const uint8_t *__restrict__ img, 
         sobel_px(img, W, H, out, x, y);
 }
 
-// Materialised cost volume (multiscale), one pixel per 16-lane row:
+// Materialised cost volume (multiscale), one pixel per 16-lane row, 16
+// consecutive pixels of one row per workgroup:
 //   merge == nullptr: C = popcount(cl ^ cr[x-d]) or `invalid`      (sgm_ms_ref.c:66-72)
 //   merge != nullptr: C = (C + merge[(y/2)*hw + x/2][d/2] + 1) >> 1 (sgm_ms_ref.c:184-188)
 template <int DPL>
@@ -76,9 +77,9 @@ __global__ __launch_bounds__(256) void k_cost(const uint64_t *__restrict__ cl,
                                               uint8_t *__restrict__ C)
 {
     const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
-    const size_t p = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 4 + g;
-    if (p < (size_t)W * H)
-        cost_px<DPL>(cl, cr, W, D, invalid, merge, C, p, l);
+    const int x = (int)blockIdx.x * 16 + (int)(threadIdx.x >> 6) * 4 + g, y = (int)blockIdx.y;
+    if (x < W)
+        cost_px<DPL>(cl, cr, W, H, D, invalid, merge, C, x, y, l);
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin by id, and each XCD has its
@@ -278,14 +279,15 @@ static int launch_wta(hipStream_t s, const uint8_t *vol, size_t dir_stride, int 
 static int launch_cost(hipStream_t s, const uint64_t *cl, const uint64_t *cr, int W, int H, int D,
                        uint32_t invalid, const uint8_t *merge, uint8_t *C)
 {
-    const size_t N = (size_t)W * H;
-    const int blocks = (int)((N + 15) / 16);
+    // the codes' byte offsets must fit the 32-bit buffer offsets
+    REQUIRE((size_t)W * H * 8 < (1ull << 31), "sgmh: %dx%d exceeds the cost kernel's 2 GiB code limit", W, H);
+    const dim3 blocks((W + 15) / 16, H);
     switch (D / 16) {
-    case 1: hipLaunchKernelGGL(k_cost<1>, dim3(blocks), dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
-    case 2: hipLaunchKernelGGL(k_cost<2>, dim3(blocks), dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
-    case 4: hipLaunchKernelGGL(k_cost<4>, dim3(blocks), dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
-    case 8: hipLaunchKernelGGL(k_cost<8>, dim3(blocks), dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
-    case 16: hipLaunchKernelGGL(k_cost<16>, dim3(blocks), dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
+    case 1: hipLaunchKernelGGL(k_cost<1>, blocks, dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
+    case 2: hipLaunchKernelGGL(k_cost<2>, blocks, dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
+    case 4: hipLaunchKernelGGL(k_cost<4>, blocks, dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
+    case 8: hipLaunchKernelGGL(k_cost<8>, blocks, dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
+    case 16: hipLaunchKernelGGL(k_cost<16>, blocks, dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
     default: set_err("sgmh: bad D %d", D); return -1;
     }
     CK(hipGetLastError());
