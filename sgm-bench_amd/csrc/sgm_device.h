@@ -226,57 +226,76 @@ __device__ __forceinline__ uint64_t ld64(rsrc_t r, uint32_t off)
 // indices: a 16-byte load straddling offset 0 would be out of range as a
 // whole.  The merge terms of a lane are DPL/2 consecutive bytes, one load.
 template <int DPL>
+struct CostIn {
+    uint64_t a;       // left code
+    uint64_t b[DPL];  // b[j] = cr[row + x - l*DPL - j]
+    uint64_t mw;      // merge bytes (merge != nullptr)
+
+    __device__ __forceinline__ void load(const uint64_t *__restrict__ cl, const uint64_t *__restrict__ cr,
+                                         rsrc_t rr, int W, int D, const uint8_t *__restrict__ merge, int x,
+                                         int y, int l)
+    {
+        const uint32_t p = (uint32_t)(y * W + x);
+        a = cl[p];
+        const int lo = x - l * DPL - (DPL - 1); // lowest code of the window, relative to the row start
+        const uint32_t base = (uint32_t)(y * W + lo) * 8u;
+        if (DPL >= 2 && (y > 0 || lo >= 0)) {
+#pragma unroll
+            for (int i = 0; i < DPL; i += 2) {
+                const auto q = __builtin_amdgcn_raw_buffer_load_b128(rr, base + 8u * i, 0, 0);
+                b[DPL - 1 - i] = ((uint64_t)q[1] << 32) | q[0];
+                b[DPL - 2 - i] = ((uint64_t)q[3] << 32) | q[2];
+            }
+        } else {
+            // row 0, window starting left of x = 0: clamped in-range loads
+            // (codes left of x = 0 are replaced by `invalid`)
+            const uint64_t *crow = cr + (size_t)y * W;
+#pragma unroll
+            for (int j = 0; j < DPL; j++) {
+                const int xr = x - (l * DPL + j);
+                b[j] = crow[xr < 0 ? 0 : xr];
+            }
+        }
+        mw = 0;
+        if (merge) {
+            const uint8_t *m = merge + ((size_t)(y / 2) * (W / 2) + x / 2) * D;
+            if constexpr (DPL == 1)
+                mw = m[l >> 1];
+            else if constexpr (DPL == 2)
+                mw = m[l];
+            else if constexpr (DPL == 4)
+                mw = *reinterpret_cast<const uint16_t *>(m + 2 * l);
+            else if constexpr (DPL == 8)
+                mw = *reinterpret_cast<const uint32_t *>(m + 4 * l);
+            else
+                mw = *reinterpret_cast<const uint64_t *>(m + 8 * l);
+        }
+    }
+
+    __device__ __forceinline__ void finish(int W, int D, uint32_t invalid, bool merged, uint8_t *__restrict__ C,
+                                           int x, int y, int l) const
+    {
+        uint32_t c[DPL];
+#pragma unroll
+        for (int j = 0; j < DPL; j++) {
+            const uint32_t h = (uint32_t)__popcll(a ^ b[j]);
+            c[j] = x - (l * DPL + j) < 0 ? invalid : h;
+            if (merged)
+                c[j] = (c[j] + (uint32_t)((mw >> (8 * (j >> 1))) & 0xFFu) + 1) >> 1;
+        }
+        store_bytes<DPL>(C + (size_t)(y * W + x) * D + l * DPL, c);
+    }
+};
+
+template <int DPL>
 __device__ __forceinline__ void cost_px(const uint64_t *__restrict__ cl, const uint64_t *__restrict__ cr,
                                         int W, int H, int D, uint32_t invalid, const uint8_t *__restrict__ merge,
                                         uint8_t *__restrict__ C, int x, int y, int l)
 {
-    const uint32_t p = (uint32_t)(y * W + x);
-    const uint64_t a = cl[p];
     const rsrc_t rr = make_rsrc(cr, (uint32_t)(W * H) * 8u);
-    const int lo = x - l * DPL - (DPL - 1); // lowest code of the window, relative to the row start
-    uint64_t b[DPL];                        // b[j] = cr[row + x - l*DPL - j]
-    const uint32_t base = (uint32_t)(y * W + lo) * 8u;
-    if (DPL >= 2 && (y > 0 || lo >= 0)) {
-#pragma unroll
-        for (int i = 0; i < DPL; i += 2) {
-            const auto q = __builtin_amdgcn_raw_buffer_load_b128(rr, base + 8u * i, 0, 0);
-            b[DPL - 1 - i] = ((uint64_t)q[1] << 32) | q[0];
-            b[DPL - 2 - i] = ((uint64_t)q[3] << 32) | q[2];
-        }
-    } else {
-        // row 0, window starting left of x = 0: clamped in-range loads (codes
-        // left of x = 0 are replaced by `invalid` below)
-        const uint64_t *crow = cr + (size_t)y * W;
-#pragma unroll
-        for (int j = 0; j < DPL; j++) {
-            const int xr = x - (l * DPL + j);
-            b[j] = crow[xr < 0 ? 0 : xr];
-        }
-    }
-    uint32_t c[DPL];
-#pragma unroll
-    for (int j = 0; j < DPL; j++) {
-        const uint32_t h = (uint32_t)__popcll(a ^ b[j]);
-        c[j] = x - (l * DPL + j) < 0 ? invalid : h;
-    }
-    if (merge) {
-        const uint8_t *m = merge + ((size_t)(y / 2) * (W / 2) + x / 2) * D;
-        uint64_t mw;
-        if constexpr (DPL == 1)
-            mw = m[l >> 1];
-        else if constexpr (DPL == 2)
-            mw = m[l];
-        else if constexpr (DPL == 4)
-            mw = *reinterpret_cast<const uint16_t *>(m + 2 * l);
-        else if constexpr (DPL == 8)
-            mw = *reinterpret_cast<const uint32_t *>(m + 4 * l);
-        else
-            mw = *reinterpret_cast<const uint64_t *>(m + 8 * l);
-#pragma unroll
-        for (int j = 0; j < DPL; j++)
-            c[j] = (c[j] + (uint32_t)((mw >> (8 * (j >> 1))) & 0xFFu) + 1) >> 1;
-    }
-    store_bytes<DPL>(C + (size_t)p * D + l * DPL, c);
+    CostIn<DPL> in;
+    in.load(cl, cr, rr, W, D, merge, x, y, l);
+    in.finish(W, D, invalid, merge != nullptr, C, x, y, l);
 }
 
 // Store DPL packed bytes at byte offset `off` (kDrop: no store).  NT: the

@@ -66,8 +66,11 @@ __global__ __launch_bounds__(256) void k_sobel(const uint8_t *__restrict__ img, 
         sobel_px(img, W, H, out, x, y);
 }
 
-// Materialised cost volume (multiscale), one pixel per 16-lane row, 16
-// consecutive pixels of one row per workgroup:
+constexpr int kCostBatch = 4; // pixel groups per wave of the cost kernel
+
+// Materialised cost volume (multiscale), one pixel per 16-lane row, 64
+// consecutive pixels of one row per workgroup; a wave takes 4 pixels 16 apart
+// at a time, kCostBatch times, with every load issued before any result:
 //   merge == nullptr: C = popcount(cl ^ cr[x-d]) or `invalid`      (sgm_ms_ref.c:66-72)
 //   merge != nullptr: C = (C + merge[(y/2)*hw + x/2][d/2] + 1) >> 1 (sgm_ms_ref.c:184-188)
 template <int DPL>
@@ -77,9 +80,21 @@ __global__ __launch_bounds__(256) void k_cost(const uint64_t *__restrict__ cl,
                                               uint8_t *__restrict__ C)
 {
     const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
-    const int x = (int)blockIdx.x * 16 + (int)(threadIdx.x >> 6) * 4 + g, y = (int)blockIdx.y;
-    if (x < W)
-        cost_px<DPL>(cl, cr, W, H, D, invalid, merge, C, x, y, l);
+    const int x0 = (int)blockIdx.x * (16 * kCostBatch) + (int)(threadIdx.x >> 6) * 4 + g, y = (int)blockIdx.y;
+    const rsrc_t rr = make_rsrc(cr, (uint32_t)(W * H) * 8u);
+    CostIn<DPL> in[kCostBatch];
+#pragma unroll
+    for (int u = 0; u < kCostBatch; u++) {
+        const int x = x0 + 16 * u;
+        if (x < W)
+            in[u].load(cl, cr, rr, W, D, merge, x, y, l);
+    }
+#pragma unroll
+    for (int u = 0; u < kCostBatch; u++) {
+        const int x = x0 + 16 * u;
+        if (x < W)
+            in[u].finish(W, D, invalid, merge != nullptr, C, x, y, l);
+    }
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin by id, and each XCD has its
@@ -281,7 +296,7 @@ static int launch_cost(hipStream_t s, const uint64_t *cl, const uint64_t *cr, in
 {
     // the codes' byte offsets must fit the 32-bit buffer offsets
     REQUIRE((size_t)W * H * 8 < (1ull << 31), "sgmh: %dx%d exceeds the cost kernel's 2 GiB code limit", W, H);
-    const dim3 blocks((W + 15) / 16, H);
+    const dim3 blocks((W + 16 * kCostBatch - 1) / (16 * kCostBatch), H);
     switch (D / 16) {
     case 1: hipLaunchKernelGGL(k_cost<1>, blocks, dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
     case 2: hipLaunchKernelGGL(k_cost<2>, blocks, dim3(256), 0, s, cl, cr, W, H, D, invalid, merge, C); break;
