@@ -222,13 +222,16 @@ static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndi
     for (int k = 0; k < ndirs; k++)
         order[k] = k;
     auto len = [&](int k) { return dirs[k][1] == 0 ? a.W : (dirs[k][0] == 0 ? a.H : std::min(a.W, a.H)); };
-    // Census-fed plain paths put the row directions last.  The hardware places
-    // single-wave workgroups on SIMDs by wave count, not by role: with the rows
-    // first it put two row waves on one SIMD in some frames (3-30 % of them,
-    // depending on the kernel that ran before), and such a frame's paths take
-    // 0.345 instead of 0.26 ms at 1080p.  Placed after the columns, the rows
-    // never shared a SIMD in the frames measured, at the same median time.
-    if (from_census && !sat && !decimate)
+    // Census-fed 4-path launches put the row directions last.  The hardware
+    // places single-wave workgroups on SIMDs by wave count, not by role: with
+    // the rows first it put two row waves on one SIMD in some frames (3-30 %
+    // of them, depending on the kernel that ran before), and such a frame's
+    // paths take 0.345 instead of 0.26 ms at 1080p.  Placed after the columns,
+    // the rows never shared a SIMD in the frames measured, at the same median
+    // time.  With 8 paths the launch holds more waves than the SIMDs take at
+    // once, and rows placed last would wait for a slot: those keep the rows
+    // first.
+    if (from_census && !sat && !decimate && ndirs <= 4)
         std::stable_sort(order, order + ndirs, [&](int p, int q) { return (dirs[p][1] == 0) < (dirs[q][1] == 0); });
     else
         std::stable_sort(order, order + ndirs, [&](int p, int q) { return len(p) > len(q); });
