@@ -1353,7 +1353,8 @@ __device__ __forceinline__ void wta_window_tile(const uint8_t *__restrict__ L, s
         const int b = b0 + (lane & 1);
         const bool act = b <= b1;
 #pragma unroll
-        for (int it = 2 * part; it < 2 * part + 2; it++) {
+        for (int i = 0; i < 2; i++) {
+            const int it = 2 * part + i;
             const int r = it * 32 + (lane >> 1), y = ty * 16 + (r >> 4), x = tx * 16 + (r & 15);
             const bool valid = y < H && x < W;
             const size_t p = valid ? (size_t)y * W + x : 0;
@@ -1369,7 +1370,8 @@ __device__ __forceinline__ void wta_window_tile(const uint8_t *__restrict__ L, s
         const int sub = lane % NB;
         const bool act = sub >= b0 && sub <= b1;
 #pragma unroll 2
-        for (int it = PASSES * part; it < PASSES * part + PASSES; it++) {
+        for (int i = 0; i < PASSES; i++) {
+            const int it = PASSES * part + i;
             const int r = it * PPW + lane / NB, y = ty * 16 + (r >> 4), x = tx * 16 + (r & 15);
             const bool valid = y < H && x < W;
             const size_t p = valid ? (size_t)y * W + x : 0;
