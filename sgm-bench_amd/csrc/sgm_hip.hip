@@ -205,10 +205,10 @@ static int launch_path_t(hipStream_t s, const PathArgs &a, int blocks)
     return 0;
 }
 
-// Fill the direction table and launch: census-fed plain paths with the row
-// directions last (see below), the others with the longest lines first, so
-// the first blocks the dispatcher places are the longest-running ones.  Two
-// variants are instantiated: census-fed plain paths and cost-volume-fed
+// Fill the direction table and launch: census-fed 4-path launches with the
+// row directions last (see below), the others with the longest lines first,
+// so the first blocks the dispatcher places are the longest-running ones.
+// Two variants are instantiated: census-fed plain paths and cost-volume-fed
 // saturating multiscale paths (full or decimated store).
 static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndirs,
                         bool from_census, bool sat, uint8_t *vol, size_t dir_stride, int decimate)
