@@ -629,7 +629,9 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
         fetch(pix + k * pstep, k);
         __builtin_amdgcn_sched_barrier(0); // slot 0 first: its wait is the shortest
     }
-    uint32_t L[DPL], mn = 0;
+    // PK: the multiscale D = 64 step on uint16 pairs (sgm_step_sat_pk)
+    constexpr bool PK = !FROM_CENSUS && SAT && DPL == 4;
+    uint32_t L[DPL], mn = 0, Lw[2] = {0u, 0u};
 #pragma unroll
     for (int j = 0; j < DPL; j++)
         L[j] = 0;
@@ -655,15 +657,23 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
                     else
                         C[j] = rel >= j ? h : inval;
                 }
-            } else {
+            } else if constexpr (!PK) {
 #pragma unroll
                 for (int j = 0; j < DPL; j++)
                     C[j] = (rw[k].w[j / 4] >> (8 * (j % 4))) & 0xFFu;
             }
+            uint32_t Cw[2];
+            if constexpr (PK) { // the 4 cost bytes as two uint16 pairs
+                Cw[0] = __builtin_amdgcn_perm(0u, rw[k].w[0], 0x0C010C00u);
+                Cw[1] = __builtin_amdgcn_perm(0u, rw[k].w[0], 0x0C030C02u);
+            }
             // ring slot k is dead now: refill it for step s + PF in place
             __builtin_amdgcn_sched_barrier(0);
             fetch(pix + PF * pstep, k);
-            sgm_step<DPL, SAT>(L, C, mn, P1, P2);
+            if constexpr (PK)
+                sgm_step_sat_pk(Lw, Cw, mn, P1, P2);
+            else
+                sgm_step<DPL, SAT>(L, C, mn, P1, P2);
             uint32_t off;
             if constexpr (!DECIM) {
                 off = (uint32_t)pix * (uint32_t)D + (uint32_t)lbase;
@@ -674,7 +684,11 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
             }
             if constexpr (CHECK)
                 off = s < steps ? off : kDrop;
-            st_bytes<DPL, (DPL >= 8)>(ro, off | dead, L);
+            if constexpr (PK) // bytes 0 and 2 of each pair word
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_amdgcn_perm(Lw[1], Lw[0], 0x06040200u), ro,
+                                                      off | dead, 0, 0);
+            else
+                st_bytes<DPL, (DPL >= 8)>(ro, off | dead, L);
             pix += pstep;
             x += dx;
             y += dy;
