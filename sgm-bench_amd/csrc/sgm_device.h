@@ -79,23 +79,50 @@ __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uin
 {
     uint16_t *t16 = reinterpret_cast<uint16_t *>(tile);
     // every staging load is issued before the first LDS write, so a thread
-    // waits for one memory latency rather than one per byte
-    constexpr int kBytes = kCenRows * kCenRowBytes, kIters = (kBytes + 255) / 256;
-    uint32_t px[kIters];
+    // waits for one memory latency rather than one per load.  A tile whose
+    // columns x0-4 .. x0+67 all lie inside a row, of a width that is a multiple
+    // of 4 (and an image base aligned to 4 bytes) is staged by aligned 4-byte
+    // words; the others by clamped bytes.
+    if (x0 >= 4 && x0 + kCenTX + 4 <= W && (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0) {
+        constexpr int kWords = kCenRows * (kCenRowBytes / 4), kIters = (kWords + 255) / 256;
+        uint32_t wv[kIters];
 #pragma unroll
-    for (int k = 0; k < kIters; k++) {
-        const int i = (int)threadIdx.x + 256 * k;
-        const int r = i / kCenRowBytes, c = i - r * kCenRowBytes;
-        const int yy = clampi(y0 - 3 + r, 0, H - 1), xx = clampi(x0 - 4 + c, 0, W - 1);
-        if (k + 1 < kIters || i < kBytes)
-            px[k] = img[(size_t)yy * W + xx];
-    }
-    __syncthreads(); // the previous tile of a persistent loop is no longer read
+        for (int k = 0; k < kIters; k++) {
+            const int i = (int)threadIdx.x + 256 * k;
+            const int r = i / (kCenRowBytes / 4), q = i - r * (kCenRowBytes / 4);
+            const int yy = clampi(y0 - 3 + r, 0, H - 1);
+            if (k + 1 < kIters || i < kWords)
+                wv[k] = *reinterpret_cast<const uint32_t *>(img + (size_t)yy * W + (x0 - 4 + 4 * q));
+        }
+        __syncthreads(); // the previous tile of a persistent loop is no longer read
 #pragma unroll
-    for (int k = 0; k < kIters; k++) {
-        const int i = (int)threadIdx.x + 256 * k;
-        if (k + 1 < kIters || i < kBytes)
-            t16[i] = (uint16_t)px[k];
+        for (int k = 0; k < kIters; k++) {
+            const int i = (int)threadIdx.x + 256 * k;
+            if (k + 1 < kIters || i < kWords) {
+                // 4 bytes -> 4 uint16: pixels 4i .. 4i+3 of the tile
+                const uint2 v = {__builtin_amdgcn_perm(0u, wv[k], 0x0C010C00u),
+                                 __builtin_amdgcn_perm(0u, wv[k], 0x0C030C02u)};
+                reinterpret_cast<uint2 *>(tile)[i] = v;
+            }
+        }
+    } else {
+        constexpr int kBytes = kCenRows * kCenRowBytes, kIters = (kBytes + 255) / 256;
+        uint32_t px[kIters];
+#pragma unroll
+        for (int k = 0; k < kIters; k++) {
+            const int i = (int)threadIdx.x + 256 * k;
+            const int r = i / kCenRowBytes, c = i - r * kCenRowBytes;
+            const int yy = clampi(y0 - 3 + r, 0, H - 1), xx = clampi(x0 - 4 + c, 0, W - 1);
+            if (k + 1 < kIters || i < kBytes)
+                px[k] = img[(size_t)yy * W + xx];
+        }
+        __syncthreads(); // the previous tile of a persistent loop is no longer read
+#pragma unroll
+        for (int k = 0; k < kIters; k++) {
+            const int i = (int)threadIdx.x + 256 * k;
+            if (k + 1 < kIters || i < kBytes)
+                t16[i] = (uint16_t)px[k];
+        }
     }
     __syncthreads();
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
