@@ -656,9 +656,13 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
         fetch(pix + k * pstep, k);
         __builtin_amdgcn_sched_barrier(0); // slot 0 first: its wait is the shortest
     }
-    // PK: the multiscale D = 64 step on uint16 pairs (sgm_step_sat_pk)
-    constexpr bool PK = !FROM_CENSUS && SAT && DPL == 4;
-    uint32_t L[DPL], mn = 0, Lw[2] = {0u, 0u};
+    // PK: the multiscale step on uint16 pairs at D = 64 and 128 (sgm_step_sat_pk)
+    constexpr bool PK = !FROM_CENSUS && SAT && (DPL == 4 || DPL == 8);
+    constexpr int NW = DPL >= 2 ? DPL / 2 : 1; // pair words (used when PK)
+    uint32_t L[DPL], mn = 0, Lw[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++)
+        Lw[w] = 0;
 #pragma unroll
     for (int j = 0; j < DPL; j++)
         L[j] = 0;
@@ -689,16 +693,19 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
                 for (int j = 0; j < DPL; j++)
                     C[j] = (rw[k].w[j / 4] >> (8 * (j % 4))) & 0xFFu;
             }
-            uint32_t Cw[2];
-            if constexpr (PK) { // the 4 cost bytes as two uint16 pairs
-                Cw[0] = __builtin_amdgcn_perm(0u, rw[k].w[0], 0x0C010C00u);
-                Cw[1] = __builtin_amdgcn_perm(0u, rw[k].w[0], 0x0C030C02u);
+            uint32_t Cw[NW];
+            if constexpr (PK) { // the cost bytes as uint16 pairs
+#pragma unroll
+                for (int q = 0; q < DPL / 4; q++) {
+                    Cw[2 * q] = __builtin_amdgcn_perm(0u, rw[k].w[q], 0x0C010C00u);
+                    Cw[2 * q + 1] = __builtin_amdgcn_perm(0u, rw[k].w[q], 0x0C030C02u);
+                }
             }
             // ring slot k is dead now: refill it for step s + PF in place
             __builtin_amdgcn_sched_barrier(0);
             fetch(pix + PF * pstep, k);
             if constexpr (PK)
-                sgm_step_sat_pk(Lw, Cw, mn, P1, P2);
+                sgm_step_sat_pk<NW>(Lw, Cw, mn, P1, P2);
             else
                 sgm_step<DPL, SAT>(L, C, mn, P1, P2);
             uint32_t off;
@@ -711,10 +718,15 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
             }
             if constexpr (CHECK)
                 off = s < steps ? off : kDrop;
-            if constexpr (PK) // bytes 0 and 2 of each pair word
+            if constexpr (PK && DPL == 4) { // bytes 0 and 2 of each pair word
                 __builtin_amdgcn_raw_buffer_store_b32(__builtin_amdgcn_perm(Lw[1], Lw[0], 0x06040200u), ro,
                                                       off | dead, 0, 0);
-            else
+            } else if constexpr (PK) { // D = 128: 8 bytes, non-temporal like st_bytes at 8 per lane
+                typedef uint32_t v2u __attribute__((ext_vector_type(2)));
+                const v2u q = {__builtin_amdgcn_perm(Lw[1], Lw[0], 0x06040200u),
+                               __builtin_amdgcn_perm(Lw[3], Lw[2], 0x06040200u)};
+                __builtin_amdgcn_raw_buffer_store_b64(q, ro, off | dead, 0, kStoreNT);
+            } else
                 st_bytes<DPL, (DPL >= 8)>(ro, off | dead, L);
             pix += pstep;
             x += dx;

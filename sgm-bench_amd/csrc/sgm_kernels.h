@@ -102,13 +102,14 @@ __device__ __forceinline__ void sgm_step(uint32_t (&L)[DPL], const uint32_t (&C)
     mn = row_min16(lo);
 }
 
-// The saturating step at D = 64 on uint16 pairs: lane l's disparities 4l..4l+3
-// are Lw[0] = (L(4l), L(4l+1)) and Lw[1] = (L(4l+2), L(4l+3)), low half first,
-// and Cw likewise.  Same rule as sgm_step<4, true>: every intermediate stays
-// below 2^16 (L <= 255, P1 <= 255, mn + P2 <= 455, C + m <= 255 + 455), and
-// m(d) >= mn, so C + m - mn does not wrap.  The neighbours d-1 / d+1 inside
-// a lane are one v_alignbit_b32 of the two words; across lanes they come by
-// the same row_shr / row_shl DPP moves (lane 0 / 15: 0xFFFF, none).
+// The saturating step on uint16 pairs (D = 32 * NW): lane l's disparities
+// l*2NW .. l*2NW+2NW-1 are Lw[w] = (L(l*2NW+2w), L(l*2NW+2w+1)), low half
+// first, and Cw likewise.  Same rule as sgm_step<2*NW, true>: every
+// intermediate stays below 2^16 (L <= 255, P1 <= 255, mn + P2 <= 455,
+// C + m <= 255 + 455), and m(d) >= mn, so C + m - mn does not wrap.  The
+// neighbours d-1 / d+1 inside a lane are one v_alignbit_b32 of two adjacent
+// words; across lanes they come by the same row_shr / row_shl DPP moves
+// (lane 0 / 15: 0xFFFF, none).
 typedef uint16_t u16x2_t __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t pk_u16(u16x2_t v)
@@ -131,25 +132,34 @@ __device__ __forceinline__ uint32_t pk_min_u16(uint32_t a, uint32_t b)
     return pk_u16(x < y ? x : y);
 }
 
-__device__ __forceinline__ void sgm_step_sat_pk(uint32_t (&Lw)[2], const uint32_t (&Cw)[2], uint32_t &mn,
+template <int NW>
+__device__ __forceinline__ void sgm_step_sat_pk(uint32_t (&Lw)[NW], const uint32_t (&Cw)[NW], uint32_t &mn,
                                                 uint32_t P1, uint32_t P2)
 {
     const uint32_t p1 = P1 | (P1 << 16);
-    const uint32_t a0 = pk_u16(as_u16x2(Lw[0]) + as_u16x2(p1));
-    const uint32_t a1 = pk_u16(as_u16x2(Lw[1]) + as_u16x2(p1));
-    const uint32_t below = dpp_or<kRowShr1>(0xFFFFFFFFu, a1); // high half: Lp(4l-1)+P1
-    const uint32_t above = dpp_or<kRowShl1>(0xFFFFFFFFu, a0); // low half: Lp(4l+4)+P1
-    const uint32_t lo0 = __builtin_amdgcn_alignbit(a0, below, 16); // (d-1 of 4l, of 4l+1)
-    const uint32_t mid = __builtin_amdgcn_alignbit(a1, a0, 16);    // (4l+1, 4l+2)
-    const uint32_t hi1 = __builtin_amdgcn_alignbit(above, a1, 16); // (4l+3, 4l+4)
+    uint32_t a[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++)
+        a[w] = pk_u16(as_u16x2(Lw[w]) + as_u16x2(p1));
+    const uint32_t below = dpp_or<kRowShr1>(0xFFFFFFFFu, a[NW - 1]); // high half: Lp(first d - 1)+P1
+    const uint32_t above = dpp_or<kRowShl1>(0xFFFFFFFFu, a[0]);      // low half: Lp(last d + 1)+P1
+    // nb[w]: the d-1 neighbours of word w, which are the d+1 neighbours of word w-1
+    uint32_t nb[NW + 1];
+    nb[0] = __builtin_amdgcn_alignbit(a[0], below, 16);
+#pragma unroll
+    for (int w = 1; w < NW; w++)
+        nb[w] = __builtin_amdgcn_alignbit(a[w], a[w - 1], 16);
+    nb[NW] = __builtin_amdgcn_alignbit(above, a[NW - 1], 16);
     const uint32_t jump = mn + P2, jump2 = jump | (jump << 16), mn2 = mn | (mn << 16);
-    const uint32_t m0 = pk_min_u16(pk_min_u16(Lw[0], jump2), pk_min_u16(lo0, mid));
-    const uint32_t m1 = pk_min_u16(pk_min_u16(Lw[1], jump2), pk_min_u16(mid, hi1));
     const uint32_t sat = 255u | (255u << 16);
-    Lw[0] = pk_min_u16(pk_u16(as_u16x2(Cw[0]) + as_u16x2(m0) - as_u16x2(mn2)), sat);
-    Lw[1] = pk_min_u16(pk_u16(as_u16x2(Cw[1]) + as_u16x2(m1) - as_u16x2(mn2)), sat);
-    const uint32_t w = pk_min_u16(Lw[0], Lw[1]);
-    mn = row_min16(umin(w & 0xFFFFu, w >> 16));
+    uint32_t lo = 0xFFFFFFFFu;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        const uint32_t m = pk_min_u16(pk_min_u16(Lw[w], jump2), pk_min_u16(nb[w], nb[w + 1]));
+        Lw[w] = pk_min_u16(pk_u16(as_u16x2(Cw[w]) + as_u16x2(m) - as_u16x2(mn2)), sat);
+        lo = pk_min_u16(lo, Lw[w]);
+    }
+    mn = row_min16(umin(lo & 0xFFFFu, lo >> 16));
 }
 
 // Four byte values (each < 256) into one word: two v_lshl_or_b32 and one
