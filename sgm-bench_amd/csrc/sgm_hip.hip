@@ -224,9 +224,9 @@ static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndi
     auto len = [&](int k) { return dirs[k][1] == 0 ? a.W : (dirs[k][0] == 0 ? a.H : std::min(a.W, a.H)); };
     // Census-fed 4-path launches put the row directions last.  The hardware
     // places single-wave workgroups on SIMDs by wave count, not by role: with
-    // the rows first it put two row waves on one SIMD in some frames (3-30 %
-    // of them, depending on the kernel that ran before), and such a frame's
-    // paths take 0.345 instead of 0.26 ms at 1080p.  Placed after the columns,
+    // the rows first it put two row waves on one SIMD in some frames (from a
+    // few percent of them to nearly all, depending on the kernel that ran
+    // before), and such a frame's paths take 0.345 instead of 0.26 ms at 1080p.  Placed after the columns,
     // the rows never shared a SIMD in the frames measured, at the same median
     // time.  With 8 paths the launch holds more waves than the SIMDs take at
     // once, and rows placed last would wait for a slot: those keep the rows
