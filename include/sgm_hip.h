@@ -82,6 +82,22 @@ int sgmh_plain(sgmh_ctx *ctx, const uint8_t *left, const uint8_t *right,
 int sgmh_plain_dev(sgmh_ctx *ctx, const uint8_t *d_left, const uint8_t *d_right,
                    int W, int H, int D, int paths, uint8_t *d_disp, void *stream);
 
+/* No counterpart in the reference: sgmh_plain_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same semantics and output as
+ * sgmh_plain_dev, with no context: the census codes and direction volumes live
+ * in one static device array of SGMH_NOALLOC_SCRATCH_BYTES per GPU (each GPU
+ * the library runs on holds it until the process ends), resolved on the
+ * current device (hipSetDevice) at the first call there or at
+ * sgmh_ctx_create.  It needs 2 * align256((W*H + 256) * 8) + paths*W*H*D bytes
+ * (640x480 D=64 4-path: 83.6 MB) and returns -1 for frames that need more.
+ * It calls no hipMalloc / hipFree.  Every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (other streams, other threads)
+ * race; calls on one stream are ordered and safe. */
+#define SGMH_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int sgmh_plain_noalloc_dev(const uint8_t *d_left, const uint8_t *d_right, int W, int H, int D,
+                           int paths, uint8_t *d_disp, void *stream);
+
 /* ---- multiscale (Configurations B/C): replaces sgm_ms_ref.c:166-202 ---- */
 
 /* disp is (W/2)*(H/2).  disp1 (may be NULL) receives the stage-1 half-resolution

@@ -19,6 +19,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "sgm_device.h"
@@ -348,18 +349,22 @@ static int ensure_codes(DevBuf &b, size_t N)
     return 0;
 }
 
-static int plain_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W,
-                         int H, int D, int paths, uint8_t *disp)
+// The plain path's device workspace: the two census code buffers, each
+// kLead + N codes (the first kLead zero), and the direction volumes,
+// paths * N * D bytes.
+struct PlainWork {
+    uint64_t *cl, *cr; // padded bases
+    uint8_t *vol;
+};
+
+// The plain pipeline on explicit workspaces (ctx may be NULL: then nothing is timed).
+static int plain_launch(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W, int H,
+                        int D, int paths, const PlainWork &w, uint8_t *disp)
 {
     const size_t N = (size_t)W * H;
-    if (ensure_codes(ctx->cl, N) || ensure_codes(ctx->cr, N) ||
-        ensure(ctx->vol, N * D * paths))
-        return -1;
-    uint64_t *cl = (uint64_t *)ctx->cl.p, *cr = (uint64_t *)ctx->cr.p; // padded bases
-    uint8_t *vol = (uint8_t *)ctx->vol.p;
     {
         PhaseScope ps(ctx, s, PH_CENSUS);
-        if (launch_census(s, l, cl + kLead, W, H, W, r, cr + kLead))
+        if (launch_census(s, l, w.cl + kLead, W, H, W, r, w.cr + kLead))
             return -1;
     }
     {
@@ -370,16 +375,73 @@ static int plain_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const u
             dirs[k][1] = kDirsPlain[k][1];
             dirs[k][2] = (int)kPlainP1;
         }
-        if (plain_paths(ctx, s, cl, cr, W, H, D, dirs, paths, vol, N * D))
+        if (plain_paths(ctx, s, w.cl, w.cr, W, H, D, dirs, paths, w.vol, N * D))
             return -1;
     }
     {
         PhaseScope ps(ctx, s, PH_ARGMIN);
-        if (launch_wta(s, vol, N * D, paths, N, D, disp))
+        if (launch_wta(s, w.vol, N * D, paths, N, D, disp))
             return -1;
     }
     return 0;
 }
+
+static int plain_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint8_t *r, int W,
+                         int H, int D, int paths, uint8_t *disp)
+{
+    const size_t N = (size_t)W * H;
+    if (ensure_codes(ctx->cl, N) || ensure_codes(ctx->cr, N) ||
+        ensure(ctx->vol, N * D * paths))
+        return -1;
+    const PlainWork w = {(uint64_t *)ctx->cl.p, (uint64_t *)ctx->cr.p, (uint8_t *)ctx->vol.p};
+    return plain_launch(ctx, s, l, r, W, H, D, paths, w, disp);
+}
+
+// ---------------------------------------------------------------------------
+// the plain path without allocation: its workspace is carved out of one
+// static device array per GPU (for callers that cannot allocate)
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = SGMH_NOALLOC_SCRATCH_BYTES; // 128 MiB per GPU, held until the process ends
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+// The device address of `scratch` on each GPU (its name in host code is only a
+// stand-in), resolved once per device with that device current.
+static uint8_t *g_scratch_ptr[kScratchDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_on(int dev, uint8_t **out)
+{
+    REQUIRE(dev >= 0 && dev < kScratchDevices, "sgmh: device %d beyond the %d the scratch table holds", dev,
+            kScratchDevices);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        CK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        REQUIRE(p != nullptr, "sgmh: hipGetSymbolAddress gave no address for the scratch array");
+        g_scratch_ptr[dev] = (uint8_t *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Scratch layout of one no-allocation call: cl codes, cr codes, volumes.
+static size_t noalloc_bytes(size_t N, int D, int paths)
+{
+    return 2 * align256((N + kLead) * 8) + (size_t)paths * N * D;
+}
+
+// zeroes the kLead codes before pixel 0 of both code buffers (a fill kernel,
+// not a memset node, so that a captured graph replays it correctly)
+__global__ __launch_bounds__(256) void k_zero_lead(uint64_t *__restrict__ a, uint64_t *__restrict__ b)
+{
+    const int i = (int)threadIdx.x;
+    a[i] = 0;
+    b[i] = 0;
+}
+static_assert(kLead == 256, "k_zero_lead clears kLead codes with 256 threads");
 
 // The windowed path in two parts, so that the learned path can run the census
 // beside the proposal network: window_prepare sizes the workspaces (it may
@@ -548,6 +610,11 @@ int sgmh_ctx_create(sgmh_ctx **out, int device)
     CK(hipGetDeviceCount(&n));
     REQUIRE(device >= 0 && device < n, "sgmh_ctx_create: device %d of %d", device, n);
     CK(hipSetDevice(device));
+    // the no-allocation entry point's scratch address on this device, resolved
+    // here at set-up (its first call on a device resolves it otherwise)
+    uint8_t *scr = nullptr;
+    if (scratch_on(device, &scr))
+        return -1;
     sgmh_ctx *c = new sgmh_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -654,6 +721,33 @@ int sgmh_plain(sgmh_ctx *c, const uint8_t *left, const uint8_t *right, int W, in
     if (t)
         return sgmh_get_times(c, t);
     return 0;
+}
+
+int sgmh_plain_noalloc_dev(const uint8_t *l, const uint8_t *r, int W, int H, int D, int paths,
+                           uint8_t *disp, void *stream)
+{
+    REQUIRE(l && r && disp, "sgmh_plain_noalloc_dev: NULL argument");
+    if (check_dims(W, H, D))
+        return -1;
+    REQUIRE(paths == 4 || paths == 8, "sgmh_plain_noalloc_dev: paths=%d (4 or 8)", paths);
+    const size_t N = (size_t)W * H, need = noalloc_bytes(N, D, paths);
+    REQUIRE(need <= kScratchBytes,
+            "sgmh_plain_noalloc_dev: %dx%d D=%d %d-path needs %zu bytes of scratch, more than the %zu "
+            "the library holds per GPU",
+            W, H, D, paths, need, kScratchBytes);
+    int dev = 0;
+    CK(hipGetDevice(&dev));
+    uint8_t *base = nullptr;
+    if (scratch_on(dev, &base))
+        return -1;
+    hipStream_t s = (hipStream_t)stream;
+    PlainWork w;
+    w.cl = (uint64_t *)base;
+    w.cr = (uint64_t *)(base + align256((N + kLead) * 8));
+    w.vol = base + 2 * align256((N + kLead) * 8);
+    hipLaunchKernelGGL(k_zero_lead, dim3(1), dim3(256), 0, s, w.cl, w.cr);
+    CK(hipGetLastError());
+    return plain_launch(nullptr, s, l, r, W, H, D, paths, w, disp);
 }
 
 int sgmh_ms_dev(sgmh_ctx *c, const uint8_t *l, const uint8_t *r, int W, int H, int D, uint8_t *disp,

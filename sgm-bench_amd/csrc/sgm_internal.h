@@ -77,14 +77,15 @@ inline void free_buf(DevBuf &b)
     b.cap = 0;
 }
 
-// Phase timing: events are recorded only when timing is on.
+// Phase timing: events are recorded only when timing is on (and never
+// without a context).
 struct PhaseScope {
     sgmh_ctx *ctx;
     hipStream_t s;
     int phase, ev0 = -1;
     PhaseScope(sgmh_ctx *c, hipStream_t st, int ph) : ctx(c), s(st), phase(ph)
     {
-        if (ctx->timing)
+        if (ctx && ctx->timing) // no context: nothing is timed
             ev0 = record();
     }
     int record()
@@ -102,7 +103,7 @@ struct PhaseScope {
     }
     ~PhaseScope()
     {
-        if (ctx->timing && ev0 >= 0) {
+        if (ctx && ctx->timing && ev0 >= 0) {
             const int ev1 = record();
             if (ev1 >= 0)
                 ctx->marks.push_back({phase, {ev0, ev1}});

@@ -105,6 +105,7 @@ def hip_lib() -> ctypes.CDLL:
         h.sgmh_get_times.argtypes = [_P, ctypes.POINTER(Times)]
         h.sgmh_plain.argtypes = [_P, _P, _P, _I, _I, _I, _I, _P, ctypes.POINTER(Times)]
         h.sgmh_plain_dev.argtypes = [_P, _P, _P, _I, _I, _I, _I, _P, _P]
+        h.sgmh_plain_noalloc_dev.argtypes = [_P, _P, _I, _I, _I, _I, _P, _P]
         h.sgmh_ms.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P, ctypes.POINTER(Times)]
         h.sgmh_ms_dev.argtypes = [_P, _P, _P, _I, _I, _I, _P, _P, _P]
         h.sgmh_census_dev.argtypes = [_P, _P, _I, _I, _P, _P]
@@ -133,7 +134,7 @@ def hip_lib() -> ctypes.CDLL:
         h.sgmh_batch_destroy.argtypes = [_P]
         h.sgmh_batch_destroy.restype = None
         for name in ("sgmh_ctx_create", "sgmh_set_timing", "sgmh_get_times", "sgmh_plain",
-                     "sgmh_plain_dev", "sgmh_ms", "sgmh_ms_dev", "sgmh_census_dev",
+                     "sgmh_plain_dev", "sgmh_plain_noalloc_dev", "sgmh_ms", "sgmh_ms_dev", "sgmh_census_dev",
                      "sgmh_sobel_dev", "sgmh_half_dev", "sgmh_path_plain_dev", "sgmh_copy_probe",
                      "sgmh_batch_plan", "sgmh_batch_run", "sgmh_batch_run_loop",
                      "sgmh_batch_describe", "sgmh_window", "sgmh_window_dev",
@@ -143,6 +144,24 @@ def hip_lib() -> ctypes.CDLL:
             getattr(h, name).restype = _I
         _hip = h
     return _hip
+
+
+NOALLOC_SCRATCH_BYTES = 128 << 20  # SGMH_NOALLOC_SCRATCH_BYTES (include/sgm_hip.h)
+
+
+def plain_noalloc_dev(d_left: int, d_right: int, W: int, H: int, D: int, paths: int,
+                      d_disp: int, stream: int = 0) -> None:
+    """sgmh_plain_noalloc_dev: the plain path on the current device with no
+    device allocation (the library's static scratch array); no context."""
+    lib = hip_lib()
+    if lib.sgmh_plain_noalloc_dev(d_left, d_right, W, H, D, paths, d_disp, stream or None):
+        raise SgmError((lib.sgmh_last_error() or b"").decode())
+
+
+def noalloc_scratch_bytes(W: int, H: int, D: int, paths: int) -> int:
+    """Scratch bytes one sgmh_plain_noalloc_dev call needs (include/sgm_hip.h)."""
+    a = ((W * H + 256) * 8 + 255) // 256 * 256
+    return 2 * a + paths * W * H * D
 
 
 def _ptr(a: np.ndarray) -> _P:
