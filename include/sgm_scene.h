@@ -9,6 +9,9 @@
  *      built from right).  The reference goldens data/golden/synthetic.pgm,
  *      data/shared/ and data/multiscale/ were computed on its output.
  *
+ *  sgm_scene_periodic()   a seeded tie-rich scene (horizontally periodic
+ *      texture, flat patches) for the argmin's tie rule.
+ *
  *  sgm_scene_planes()     the seeded piecewise-planar generator the benchmark
  *      configurations are quoted on: left = 3x3 box-filtered uniform noise;
  *      right = left warped by a field of rectangular (optionally slanted)
@@ -46,6 +49,14 @@ typedef struct {
 
 /* gt (may be NULL) receives the disparity field in right-image coordinates. */
 int sgm_scene_planes(const sgm_scene_spec *spec, uint8_t *left, uint8_t *right, uint8_t *gt);
+
+/* Tie-rich scene: a texture repeating every `period` columns (2..64), flat
+ * rectangles, rectangular planes of integer disparity in [0, dmax] and no
+ * noise, so that many pixels' path sums have tied minima.  gt (may be NULL)
+ * receives the disparity field in right-image coordinates.  Returns 0 on
+ * success. */
+int sgm_scene_periodic(int W, int H, int dmax, int period, uint32_t seed,
+                       uint8_t *left, uint8_t *right, uint8_t *gt);
 
 /* P5 PGM I/O (maxval 255).  sgm_pgm_read returns a malloc'd buffer or NULL and
  * prints the reason to stderr (the reference's common/util.c:19-42 contract). */

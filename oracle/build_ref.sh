@@ -21,6 +21,11 @@ for D in 16 32 64 128 256; do
       "$REF/common/sgm_ref.c" "$REF/common/harness.c" "$REF/common/util.c" -lm
   gcc -O2 -w -I"$REF/common" -DSGM_D=$D -o "$OUT/gen_synthetic_d$D" "$REF/common/gen_synthetic.c"
 done
+# the reference's own tie counter (common/sgm_ref.c:14-16,125-138)
+for D in 64 128; do
+  gcc $CFL -DSGM_D=$D -DSGM_COUNT_TIES -DCFLAGS_STR='"ref"' -o "$OUT/sgm_ref_d${D}_ties" \
+      "$REF/common/sgm_ref.c" "$REF/common/harness.c" "$REF/common/util.c" -lm
+done
 gcc $CFL -o "$OUT/sgm_ms_ref" "$REF/multiscale/sgm_ms_ref.c" "$REF/common/util.c" -lm
 gcc $CFL -fopenmp -o "$OUT/sgm_ms_ref_omp" "$REF/multiscale/sgm_ms_ref.c" "$REF/common/util.c" -lm
 echo "build_ref: built $(ls "$OUT" | wc -l) binaries in $OUT"

@@ -117,6 +117,24 @@ def sgm_plain(left, right, D: int = 64, paths: int = 4, threads: int = 0) -> np.
     return out
 
 
+def path_sums(left, right, D: int = 64, paths: int = 4) -> np.ndarray:
+    """The plain path's uint16 S volume [H][W][D], summed from the per-direction
+    L volumes of orc_path_L (common/sgm_ref.c:75-99,117-118)."""
+    C = cost(census(left), census(right), D)
+    dirs = [(1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, 1), (1, -1), (-1, -1)][:paths]
+    S = np.zeros(C.shape, np.uint16)
+    for dx, dy in dirs:
+        S += path_L(C, dx, dy)
+    return S
+
+
+def tie_pixels(S) -> int:
+    """Pixels whose path-sum minimum is attained at more than one d: the
+    reference's SGM_COUNT_TIES count (common/sgm_ref.c:125-138)."""
+    S = np.asarray(S)
+    return int(((S == S.min(axis=-1, keepdims=True)).sum(axis=-1) > 1).sum())
+
+
 def sgm_ms(left, right, D: int = 64, threads: int = 0, want_stage1: bool = False):
     L, R = _c(left), _c(right)
     H, W = L.shape

@@ -243,6 +243,63 @@ int sgm_scene_planes(const sgm_scene_spec *sp, uint8_t *left, uint8_t *right, ui
 }
 
 /* ------------------------------------------------------------------------ */
+/* Tie-rich scenes (no reference counterpart)                                */
+/* ------------------------------------------------------------------------ */
+
+/* A texture that repeats every `period` columns (one random pattern per band
+ * of 8 rows, K = 6 patterns), flat rectangles of one grey level painted over
+ * it, and rectangular planes of integer disparity.  Where a plane's disparity
+ * is g, right(x) = left(x + g) with no noise, so the matching cost is the
+ * same at g, g + period, g + 2 period, ... and flat windows cost the same at
+ * every disparity: the path sums have many tied minima, which is what pins
+ * the argmin's lowest-d rule (common/sgm_ref.c:120-124). */
+int sgm_scene_periodic(int W, int H, int dmax, int period, uint32_t seed,
+                       uint8_t *left, uint8_t *right, uint8_t *gt)
+{
+    if (!left || !right || W < 16 || H < 16 || dmax < 1 || dmax > 255 || period < 2 || period > 64)
+        return -1;
+    const size_t N = (size_t)W * H;
+    uint8_t *g = gt ? gt : (uint8_t *)malloc(N);
+    if (!g)
+        return -1;
+    uint64_t s = 0x5449455343454eULL ^ (uint64_t)seed; /* "TIESCEN" ^ seed */
+    enum { K = 6 };
+    uint8_t pat[K][64];
+    for (int k = 0; k < K; k++)
+        for (int i = 0; i < period; i++)
+            pat[k][i] = (uint8_t)(sm64(&s) >> 56);
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++)
+            left[(size_t)y * W + x] = pat[(y / 8) % K][x % period];
+    const int nflat = urange(&s, 2, 4);
+    for (int k = 0; k < nflat; k++) {
+        const int rw = urange(&s, W / 10 > 1 ? W / 10 : 1, W / 4), rh = urange(&s, H / 10 > 1 ? H / 10 : 1, H / 4);
+        const int rx = urange(&s, 0, W - rw), ry = urange(&s, 0, H - rh);
+        const uint8_t v = (uint8_t)urange(&s, 16, 240);
+        for (int y = ry; y < ry + rh; y++)
+            memset(left + (size_t)y * W + rx, v, (size_t)rw);
+    }
+    memset(g, (uint8_t)urange(&s, 0, dmax / 4), N);
+    const int n = urange(&s, 3, 5);
+    for (int k = 0; k < n; k++) {
+        const int rw = urange(&s, W / 8 > 1 ? W / 8 : 1, W / 3), rh = urange(&s, H / 8 > 1 ? H / 8 : 1, H / 3);
+        const int rx = urange(&s, 0, W - rw), ry = urange(&s, 0, H - rh);
+        const uint8_t d = (uint8_t)(k == n - 1 ? dmax : urange(&s, 0, dmax));
+        for (int y = ry; y < ry + rh; y++)
+            memset(g + (size_t)y * W + rx, d, (size_t)rw);
+    }
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            const size_t i = (size_t)y * W + x;
+            const int xs = x + g[i] > W - 1 ? W - 1 : x + g[i];
+            right[i] = left[(size_t)y * W + xs];
+        }
+    if (g != gt)
+        free(g);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* PGM, hash, clock                                                          */
 /* ------------------------------------------------------------------------ */
 

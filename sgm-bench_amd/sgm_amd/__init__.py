@@ -47,6 +47,8 @@ def host_lib() -> ctypes.CDLL:
         h.sgm_scene_reference.restype = _I
         h.sgm_scene_planes.argtypes = [ctypes.POINTER(SceneSpec), _P, _P, _P]
         h.sgm_scene_planes.restype = _I
+        h.sgm_scene_periodic.argtypes = [_I, _I, _I, _I, ctypes.c_uint32, _P, _P, _P]
+        h.sgm_scene_periodic.restype = _I
         h.sgm_fnv1a64.argtypes = [_P, ctypes.c_size_t]
         h.sgm_fnv1a64.restype = ctypes.c_uint64
         h.sgm_pgm_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(_I), ctypes.POINTER(_I)]
@@ -173,6 +175,18 @@ def _u8(a) -> np.ndarray:
     return a
 
 
+def _pair(left, right):
+    """The two images of a pair as C-contiguous uint8 (H, W) arrays of one shape:
+    the library copies W*H bytes from each, so a smaller right image would be
+    read past its end."""
+    L, R = _u8(left), _u8(right)
+    if L.ndim != 2:
+        raise ValueError(f"images must be 2-D (H, W) arrays, got shape {L.shape}")
+    if R.shape != L.shape:
+        raise ValueError(f"left {L.shape} and right {R.shape} images differ in shape")
+    return L, R
+
+
 def fnv1a64(a: np.ndarray) -> str:
     """FNV-1a 64 of the bytes of `a`, as the reference prints it (%016llx)."""
     a = np.ascontiguousarray(a)
@@ -198,6 +212,16 @@ def scene_planes(W: int, H: int, dmax: int, seed: int, planes: Tuple[int, int] =
     G = np.empty_like(L)
     if host_lib().sgm_scene_planes(ctypes.byref(spec), _ptr(L), _ptr(R), _ptr(G)):
         raise SgmError("sgm_scene_planes rejected the spec")
+    return L, R, G
+
+
+def scene_periodic(W: int, H: int, dmax: int, period: int, seed: int):
+    """Tie-rich scene (include/sgm_scene.h: sgm_scene_periodic), as (H, W) arrays."""
+    L = np.empty((H, W), np.uint8)
+    R = np.empty_like(L)
+    G = np.empty_like(L)
+    if host_lib().sgm_scene_periodic(W, H, dmax, period, seed, _ptr(L), _ptr(R), _ptr(G)):
+        raise SgmError(f"sgm_scene_periodic rejected {W}x{H} dmax={dmax} period={period}")
     return L, R, G
 
 
@@ -239,6 +263,8 @@ def make_scene(spec: dict):
     """A scene from its recorded generator call (the fixtures' "scene" dicts)."""
     if spec["kind"] == "reference":
         L, R, _ = scene_reference(spec["W"], spec["H"], spec["gen_D"], spec["seed"])
+    elif spec["kind"] == "periodic":
+        L, R, _ = scene_periodic(spec["W"], spec["H"], spec["dmax"], spec["period"], spec["seed"])
     else:
         L, R, _ = scene_planes(spec["W"], spec["H"], spec["dmax"], spec["seed"],
                                planes=tuple(spec["planes"]), occl_permille=spec["occl"],
@@ -321,7 +347,7 @@ class Context:
     # ---- host-buffer entry points ----
     def plain(self, left: np.ndarray, right: np.ndarray, D: int = 64, paths: int = 4,
               times: Optional[dict] = None) -> np.ndarray:
-        L, R = _u8(left), _u8(right)
+        L, R = _pair(left, right)
         H, W = L.shape
         out = np.empty((H, W), np.uint8)
         t = Times()
@@ -333,7 +359,7 @@ class Context:
 
     def ms(self, left: np.ndarray, right: np.ndarray, D: int = 64, want_stage1: bool = False,
            times: Optional[dict] = None):
-        L, R = _u8(left), _u8(right)
+        L, R = _pair(left, right)
         H, W = L.shape
         out = np.empty((H // 2, W // 2), np.uint8)
         out1 = np.empty((H // 2, W // 2), np.uint8) if want_stage1 else None
@@ -367,9 +393,12 @@ class Context:
     def window(self, left: np.ndarray, right: np.ndarray, blk: np.ndarray, D: int = 128,
                times: Optional[dict] = None) -> np.ndarray:
         """Windowed SGM; blk[ty, tx] = first block | last block << 4."""
-        L, R = _u8(left), _u8(right)
+        L, R = _pair(left, right)
         H, W = L.shape
         blk = _u8(blk)
+        tiles = ((H + 15) // 16) * ((W + 15) // 16)
+        if blk.size != tiles:
+            raise ValueError(f"block table has {blk.size} entries; a {W}x{H} frame has {tiles} tiles")
         out = np.empty((H, W), np.uint8)
         t = Times()
         self._ck(self._lib.sgmh_window(self._h, _ptr(L), _ptr(R), W, H, D, _ptr(blk), _ptr(out),

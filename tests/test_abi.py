@@ -57,3 +57,79 @@ def test_planes_scene_deterministic():
         np.testing.assert_array_equal(x, y)
     assert not np.array_equal(a[1], c[1])
     assert a[2].max() == 63  # the last plane is pinned to dmax
+
+
+BIN = os.path.join(ROOT, "sgm-bench_amd", "bin")
+
+
+def _run(*args):
+    import subprocess
+    return subprocess.run(list(args), capture_output=True, text=True, timeout=60)
+
+
+@pytest.mark.parametrize("prog", ["sgm_hip", "sgm_hip_d128", "sgm_hip_p8", "sgm_ms_hip", "gen_scene"])
+def test_host_programs_usage(prog):
+    """Every host program prints its usage and exits 1 without arguments
+    (common/harness.c's exit code for usage and IO errors)."""
+    r = _run(os.path.join(BIN, prog))
+    assert r.returncode == 1 and "usage" in r.stderr
+
+
+def test_harness_rejects_bad_input_before_any_gpu_work(tmp_path):
+    """Unreadable files, a corrupt PGM header and mismatched sizes exit 1 on the
+    host (common/util.c:23-33, common/harness.c:142-145)."""
+    a, b, c = (str(tmp_path / n) for n in ("a.pgm", "b.pgm", "c.pgm"))
+    sgm_amd.pgm_write(a, np.zeros((20, 30), np.uint8))
+    sgm_amd.pgm_write(b, np.zeros((21, 30), np.uint8))
+    with open(c, "wb") as f:
+        f.write(b"P2\n30 20\n255\n" + bytes(600))
+    exe = os.path.join(BIN, "sgm_hip")
+    r = _run(exe, a, b)
+    assert r.returncode == 1 and "size mismatch" in r.stderr
+    r = _run(exe, a, c)
+    assert r.returncode == 1 and "not binary PGM" in r.stderr
+    r = _run(exe, a, str(tmp_path / "missing.pgm"))
+    assert r.returncode == 1 and "cannot open" in r.stderr
+    r = _run(exe, a, a, "--bogus")
+    assert r.returncode == 1 and "unknown arg" in r.stderr
+
+
+def test_gen_scene_writes_the_reference_scene(tmp_path, goldens):
+    """gen_scene reference W H DIR seed D writes the reference generator's exact
+    PGM files (sha256 of the reference's gen_synthetic output)."""
+    import hashlib
+    rec = next(r for r in goldens["reference_inputs"] if r["scene"]["W"] == 1920 and r["scene"]["H"] == 800)
+    sp = rec["scene"]
+    r = _run(os.path.join(BIN, "gen_scene"), "reference", str(sp["W"]), str(sp["H"]), str(tmp_path),
+             str(sp["seed"]), str(sp["gen_D"]))
+    assert r.returncode == 0, r.stderr
+    for n in ("left", "right", "gt"):
+        assert hashlib.sha256(open(tmp_path / (n + ".pgm"), "rb").read()).hexdigest() == rec[n + "_sha256"], n
+
+
+def test_periodic_scene_is_deterministic_and_validated(goldens):
+    for r in goldens["ties"]:
+        L, R = sgm_amd.make_scene(r["scene"])
+        assert sgm_amd.fnv1a64(np.concatenate([L.ravel(), R.ravel()])) == r["input_fnv"], r["name"]
+    with pytest.raises(sgm_amd.SgmError):
+        sgm_amd.scene_periodic(64, 64, 63, 1, 1)  # period below 2
+    with pytest.raises(sgm_amd.SgmError):
+        sgm_amd.scene_periodic(64, 64, 63, 65, 1)  # longer than the pattern buffer
+
+
+def test_host_entry_points_check_shapes_before_the_library():
+    """Context.plain / ms / window refuse a right image of another shape and a
+    block table of the wrong size before any library call (the library copies
+    W*H bytes of each image and reads one block byte per tile)."""
+    ctx = sgm_amd.Context.__new__(sgm_amd.Context)  # no device: the checks come first
+    L = np.zeros((32, 48), np.uint8)
+    with pytest.raises(ValueError, match="differ in shape"):
+        ctx.plain(L, np.zeros((32, 47), np.uint8))
+    with pytest.raises(ValueError, match="differ in shape"):
+        ctx.ms(L, np.zeros((31, 48), np.uint8))
+    with pytest.raises(ValueError, match="2-D"):
+        ctx.plain(np.zeros(100, np.uint8), np.zeros(100, np.uint8))
+    with pytest.raises(ValueError, match="tiles"):
+        ctx.window(L, L, np.zeros(5, np.uint8), 64)      # a 48x32 frame has 3x2 tiles
+    with pytest.raises(ValueError, match="differ in shape"):
+        ctx.window(L, L[:, :40], np.zeros(6, np.uint8), 64)

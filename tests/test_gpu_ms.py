@@ -29,7 +29,8 @@ def test_sobel_half(ctx, H, W):
     np.testing.assert_array_equal(d_h.cpu().numpy(), oracle.half(img))
 
 
-@pytest.mark.parametrize("H,W,D", [(32, 32, 64), (62, 98, 64), (120, 200, 64), (64, 160, 128), (40, 64, 32)])
+@pytest.mark.parametrize("H,W,D", [(32, 32, 64), (62, 98, 64), (120, 200, 64), (64, 160, 128), (40, 64, 32),
+                                   (40, 64, 16), (48, 300, 256), (200, 320, 128)])
 def test_ms_matches_oracle(ctx, H, W, D):
     L, R, _ = sgm_amd.scene_planes(W, H, D - 1 if D - 1 < W else W - 1, W + H + D, planes=(3, 6))
     got, got1 = ctx.ms(L, R, D, want_stage1=True)

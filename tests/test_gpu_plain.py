@@ -98,6 +98,24 @@ def test_computed_goldens_plain(ctx, goldens):
         assert sgm_amd.fnv1a64(out) == g["fnv"], g["name"]
 
 
+def test_tie_fixtures(ctx, goldens):
+    """Scenes where 8-43 % of pixels have tied path-sum minima (a texture that
+    repeats every 8-24 columns, flat patches): bit-exact to the compiled
+    reference, which pins the argmin's lowest-d rule (common/sgm_ref.c:120-124)
+    on the GPU, through both entry points."""
+    s = stream()
+    for g in goldens["ties"]:
+        L, R = make_scene(g["scene"])
+        assert sgm_amd.fnv1a64(np.concatenate([L.ravel(), R.ravel()])) == g["input_fnv"], g["name"]
+        out = ctx.plain(L, R, g["D"], g["paths"])
+        assert sgm_amd.fnv1a64(out) == g["fnv"], (g["name"], g["tie_pct"])
+        d_l, d_r = dev(L), dev(R)
+        d_o = torch.empty_like(d_l)
+        ctx.plain_dev(d_l.data_ptr(), d_r.data_ptr(), L.shape[1], L.shape[0], g["D"], g["paths"], d_o.data_ptr(), s)
+        torch.cuda.synchronize()
+        assert sgm_amd.fnv1a64(d_o.cpu().numpy()) == g["fnv"], g["name"]
+
+
 def test_repeat_and_resize(ctx):
     """Workspaces are reused across calls and regrown on a size change."""
     a = sgm_amd.scene_planes(300, 200, 63, 1)

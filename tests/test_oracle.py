@@ -221,6 +221,81 @@ def test_sobel_and_half_closed_forms():
     np.testing.assert_array_equal(oracle.half(img.astype(np.uint8)), want)
 
 
+# ---------------------------------------------------------------- ms composition
+# multiscale/sgm_ms_ref.c:85-87: dx, dy and the direction's P1
+MS_DIRS = [(1, 0, 40), (-1, 0, 40), (0, 1, 20), (0, -1, 20), (1, 1, 10), (-1, -1, 10), (1, -1, 10), (-1, 1, 10)]
+
+
+def sobel_closed(img):
+    p = np.pad(img.astype(np.int64), 1, mode="edge")
+    gx = (p[:-2, 2:] - p[:-2, :-2]) + 2 * (p[1:-1, 2:] - p[1:-1, :-2]) + (p[2:, 2:] - p[2:, :-2])
+    return np.clip(128 + (gx >> 3), 0, 255).astype(np.uint8)
+
+
+def half_closed(img):
+    a = img.astype(np.int64)
+    return ((a[0::2, 0::2] + a[0::2, 1::2] + a[1::2, 0::2] + a[1::2, 1::2] + 2) >> 2).astype(np.uint8)
+
+
+def cost_closed(cl, cr, D, invalid):
+    H, W = cl.shape
+    C = np.full((H, W, D), invalid, np.uint8)
+    for d in range(min(D, W)):
+        x = cl[:, d:] ^ cr[:, :W - d]
+        C[:, d:, d] = np.array([bin(int(v)).count("1") for v in x.ravel()], np.uint8).reshape(x.shape)
+    return C
+
+
+def ms_bruteforce(L, R, D):
+    """The whole multiscale composition from already-pinned pieces and the
+    reference's stage order (multiscale/sgm_ms_ref.c:166-199): half-resolution
+    SGM on half() of the raw images, Sobel-X then census at each scale, cost
+    with invalid 63, the full-resolution cost averaged with the half-resolution
+    cost at (y/2, x/2, d/2), 8 saturating weighted paths (transition form),
+    argmin of stage 1 over every half-resolution pixel and of stage 2 at
+    (2y, 2x)."""
+    def stage_cost(a, b):
+        return cost_closed(census_bruteforce(sobel_closed(a)), census_bruteforce(sobel_closed(b)), D, 63)
+
+    def S_of(C):
+        return sum(path_bruteforce(C, dx, dy, p1, 200, True).astype(np.int64) for dx, dy, p1 in MS_DIRS)
+
+    C1 = stage_cost(half_closed(L), half_closed(R))
+    disp1 = np.argmin(S_of(C1), axis=2).astype(np.uint8)   # first minimum: the lowest d
+    C2 = stage_cost(L, R).astype(np.int64)
+    H, W = L.shape
+    ys, xs = np.arange(H)[:, None], np.arange(W)[None, :]
+    C2 = ((C2 + C1[ys // 2, xs // 2][:, :, np.arange(D) // 2].astype(np.int64) + 1) >> 1).astype(np.uint8)
+    disp = np.argmin(S_of(C2)[0::2, 0::2], axis=2).astype(np.uint8)
+    return disp, disp1
+
+
+@pytest.mark.parametrize("W,H,D", [(34, 22, 16), (34, 22, 32), (34, 22, 64), (34, 22, 128), (40, 64, 32),
+                                   (40, 64, 64), (140, 18, 128), (36, 16, 256)])
+def test_ms_oracle_matches_bruteforce_composition(W, H, D):
+    """Pins orc_sgm_ms at every D the engine accepts, not only D=64 (the
+    reference's multiscale oracle hard-codes D=64, multiscale/sgm_ms_ref.c:38)."""
+    L, R, _ = sgm_amd.scene_planes(W, H, min(D - 1, W - 1), W * 7 + H + D, planes=(2, 4))
+    want, want1 = ms_bruteforce(L, R, D)
+    got, got1 = oracle.sgm_ms(L, R, D, want_stage1=True)
+    np.testing.assert_array_equal(got1, want1)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_ms_bruteforce_catches_a_merge_index_mistake():
+    """The brute force is sharp enough to see a D-dependent merge mistake: the
+    same composition merging c1[d] instead of c1[d/2] differs from it."""
+    W, H, D = 40, 24, 32
+    L, R, _ = sgm_amd.scene_planes(W, H, D - 1, 5, planes=(2, 4))
+    want, _ = ms_bruteforce(L, R, D)
+    C1 = cost_closed(census_bruteforce(sobel_closed(half_closed(L))), census_bruteforce(sobel_closed(half_closed(R))), D, 63)
+    C2 = cost_closed(census_bruteforce(sobel_closed(L)), census_bruteforce(sobel_closed(R)), D, 63).astype(np.int64)
+    ys, xs = np.arange(H)[:, None], np.arange(W)[None, :]
+    bad = ((C2 + C1[ys // 2, xs // 2].astype(np.int64) + 1) >> 1).astype(np.uint8)
+    S = sum(path_bruteforce(bad, dx, dy, p1, 200, True).astype(np.int64) for dx, dy, p1 in MS_DIRS)
+    assert not np.array_equal(np.argmin(S[0::2, 0::2], axis=2).astype(np.uint8), want)
+
+
 # ---------------------------------------------------------------- goldens
 def test_generator_matches_reference_inputs(goldens):
     import hashlib
@@ -278,3 +353,83 @@ def test_golden_discriminates(goldens):
     # a truncated search (d < 48) must fail it
     ok_trunc, _ = discriminates(np.minimum(oracle.sgm_plain(L, R, 64, 4), 47), 64)
     assert not ok_trunc
+
+
+def _all_fixtures(goldens):
+    for sec in ("reference_goldens", "computed", "ties", "batch"):
+        for r in goldens[sec]:
+            yield sec, r
+
+
+def test_every_gate_fixture_discriminates(goldens):
+    """Every golden and fixture that gates a cell passes the reference checker's
+    rule (scripts/check_golden_discriminates.py:64-119, scripts/sweep.sh:37-38);
+    the others are recorded with their role and why they are not gates."""
+    from sgm_amd.goldencheck import discriminates  # noqa: F401
+    gates = 0
+    for sec, r in _all_fixtures(goldens):
+        gc = r["goldencheck"]
+        assert gc["role"] in ("gate", "shape", "ties"), r["name"]
+        if gc["role"] == "gate":
+            gates += 1
+            assert gc["ok"], (sec, r["name"], gc)
+            # the rule it passed: a search truncated below 0.9*D leaves no pixel
+            # in the top decile, so it cannot reproduce this golden
+            assert gc["max"] == r["D"] - 1 and gc["top_decile_pct"] >= 0.5, r["name"]
+        elif gc["role"] == "shape":
+            assert gc.get("why"), r["name"]
+    assert gates >= 50
+
+
+def test_fixture_discrimination_stats_recomputed(goldens):
+    """The stored statistics are those of the map itself: recomputed from the
+    oracle's output (which the fixture hash pins) for every fixture small
+    enough to run here, and a truncated copy of each gate fails the rule."""
+    from sgm_amd.goldencheck import discriminates
+    n = 0
+    for sec, r in _all_fixtures(goldens):
+        sp = r["scene"]
+        if sec == "reference_goldens" or sp["W"] * sp["H"] * r["D"] > 640 * 480 * 128:
+            continue
+        L, R = make_scene(sp)
+        out = oracle.sgm_plain(L, R, r["D"], r["paths"]) if r["mode"] == "plain" else oracle.sgm_ms(L, R, r["D"])
+        assert oracle.fnv1a64(out) == r["fnv"], r["name"]
+        ok, st = discriminates(out, r["D"])
+        gc = r["goldencheck"]
+        assert ok == gc["ok"] and st["max"] == gc["max"] and st["distinct"] == gc["distinct"], r["name"]
+        assert st["top_decile_px"] == gc["top_decile_px"], r["name"]
+        if gc["role"] == "gate":
+            trunc = np.minimum(out, int(0.9 * r["D"]) - 1)
+            assert not discriminates(trunc, r["D"])[0], r["name"]
+        n += 1
+    assert n >= 15
+
+
+def test_tie_fixtures(goldens):
+    """Tie-rich scenes: the oracle's output hash and its count of tied-minimum
+    pixels equal what the reference's own SGM_COUNT_TIES build reported
+    (common/sgm_ref.c:125-138), and at least 1 % of pixels are tied."""
+    seen = 0
+    for r in goldens["ties"]:
+        assert r["tie_pct"] >= 1.0, r["name"]
+        sp = r["scene"]
+        if sp["W"] * sp["H"] > 640 * 480:
+            continue
+        L, R = make_scene(sp)
+        assert sgm_amd.fnv1a64(np.concatenate([L.ravel(), R.ravel()])) == r["input_fnv"], r["name"]
+        S = oracle.path_sums(L, R, r["D"], r["paths"])
+        assert oracle.tie_pixels(S) == r["tie_pixels"], r["name"]
+        disp = oracle.sgm_plain(L, R, r["D"], r["paths"])
+        np.testing.assert_array_equal(disp, np.argmin(S, axis=2))  # first minimum: lowest d
+        assert oracle.fnv1a64(disp) == r["fnv"], r["name"]
+        # the lowest-d rule is what the hash pins: taking the HIGHEST tied d
+        # changes it on these scenes
+        hi = (r["D"] - 1 - np.argmin(S[:, :, ::-1], axis=2)).astype(np.uint8)
+        assert oracle.fnv1a64(hi) != r["fnv"], r["name"]
+        seen += 1
+    assert seen >= 2
+
+
+def test_tie_count_closed_forms():
+    S = np.array([[[3, 1, 1, 5], [2, 3, 4, 5], [7, 7, 7, 7]]], np.uint16)
+    assert oracle.tie_pixels(S) == 2
