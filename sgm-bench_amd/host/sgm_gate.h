@@ -57,7 +57,7 @@ static inline int sgm_gate_calibrate(const char *path, const char *impl, const c
             fprintf(f, "%s %s %dx%d D%d %s %.6f\n", impl, board, W, H, SGM_D, sgm_gate_names[i],
                     ms[i] * 1e6 / ops[i]);
     fclose(f);
-    fprintf(stderr, "roofline: calibration written to %s for %s/%s\n", path, impl, board);
+    fprintf(stderr, "cost gate: appended the %s/%s per-phase ns/op lines to %s\n", impl, board, path);
     return 0;
 }
 
@@ -69,8 +69,8 @@ static inline int sgm_gate_check(const char *path, const char *impl, const char 
     sgm_gate_ops(W, H, t, ops, ms);
     *armed = 0;
     if (t->aggregate_ms < 0) {
-        fprintf(stderr, "ROOFLINE FAIL: aggregate_ms unreported; the ops folded into it would go "
-                        "unchecked. Report a stage time for aggregate.\n");
+        fprintf(stderr, "ROOFLINE FAIL: no aggregate time was reported, so the phases folded into "
+                        "aggregate cannot be checked (the gate fails closed).\n");
         return 1;
     }
     int any = 0;
