@@ -20,10 +20,12 @@
  *   orc_sgm_plain      the 5 on-disk reference goldens + hashes produced by the
  *                      compiled reference (oracle/_ref) on new scenes.
  *   orc_sgm_ms         the 2 on-disk multiscale goldens + compiled reference.
- *   orc_sgm_ms (D!=64) parity unpinned against the reference: the reference
- *                      hard-codes D=64 for the multiscale oracle
- *                      (multiscale/sgm_ms_ref.c:38). Pinned only by reduction to
- *                      the D=64 goldens and self-consistency.
+ *   orc_sgm_ms (D!=64) the reference hard-codes D=64 for the multiscale oracle
+ *                      (multiscale/sgm_ms_ref.c:38); every D of 16..256 is pinned
+ *                      by an independent numpy restatement of the whole
+ *                      composition (tests/test_oracle.py, ms_bruteforce).
+ *   orc_argmin ties    tie-rich scenes against the reference's SGM_COUNT_TIES
+ *                      build: its hash and its tied-pixel count.
  *   orc_sgm_window     build-defined (no reference counterpart): pinned by the
  *                      identity "all blocks active == orc_sgm_plain" and by a
  *                      brute-force transition-matrix restatement in tests.
