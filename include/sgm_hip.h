@@ -193,11 +193,16 @@ typedef struct sgmh_batch sgmh_batch;
 
 #define SGMH_BATCH_AUTOTUNE 1 /* time candidate schedules on this batch, keep the best */
 #define SGMH_BATCH_NO_GRAPH 2 /* launch the phase kernels directly, no hipGraph */
+/* Without SGMH_BATCH_AUTOTUNE, pin the two schedule choices the autotuner
+ * otherwise makes: */
+#define SGMH_BATCH_CENSUS_32X32 4 /* census in 32x32 tiles instead of 64x16 */
+#define SGMH_BATCH_XCD_SLICES 8   /* path line groups taken from per-XCD slices of the queue */
 
 /* Plan a batch: allocate every pair's workspace, build the work lists of the
  * persistent per-phase kernels (pre-filter / census, cost, path aggregation,
- * sum + argmin), optionally autotune the schedule, and capture the phases as
- * one hipGraph.  The device buffers named in `pairs` are bound into the plan:
+ * sum + argmin), optionally autotune the schedule (lane partition, path item
+ * order, path waves per SIMD, census tile shape, scanline grouping), and
+ * capture the phases as one hipGraph.  The device buffers named in `pairs` are bound into the plan:
  * later runs read whatever they then hold. */
 int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgmh_batch **out);
 /* Run the planned batch on `stream` (NULL: the HIP default stream), asynchronous. */

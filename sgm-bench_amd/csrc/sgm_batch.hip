@@ -125,11 +125,15 @@ __device__ __forceinline__ void chunk_of(int unit, int nunits, int nitems, int &
     end = min(nitems, begin + per);
 }
 
-__global__ __launch_bounds__(64) void k_batch_reset(int *counter) { if (threadIdx.x == 0) *counter = 0; }
 
+// Census tile shapes the autotuner picks between (TX x TY output pixels).
+constexpr int kCenShapes = 2;
+constexpr int kCenShapeW[kCenShapes] = {64, 32}, kCenShapeH[kCenShapes] = {16, 32};
+
+template <int TX, int TY>
 __global__ __launch_bounds__(256) void k_batch_pre(const PreTask *__restrict__ tasks, int ntasks, int nitems)
 {
-    __shared__ __align__(16) uint32_t tile[kCenRows * kCenRowWords];
+    __shared__ __align__(16) uint32_t tile[kCenTileWords];
     int begin, end;
     chunk_of(blockIdx.x, gridDim.x, nitems, begin, end);
     if (begin >= end)
@@ -142,7 +146,7 @@ __global__ __launch_bounds__(256) void k_batch_pre(const PreTask *__restrict__ t
         const int local = item - tk.first_item;
         const int tx = local % tk.tiles_x, ty = local / tk.tiles_x;
         if (tk.kind == PRE_CENSUS) {
-            census_tile(tk.src, (uint64_t *)tk.dst, tk.W, tk.H, tk.ostride, tx * kCenTX, ty * kCenTY, tile);
+            census_tile<TX, TY>(tk.src, (uint64_t *)tk.dst, tk.W, tk.H, tk.ostride, tx * TX, ty * TY, tile);
         } else {
             const int x = tx * 64 + (threadIdx.x & 63), y = ty * 4 + (threadIdx.x >> 6);
             if (tk.kind == PRE_HALF) {
@@ -180,6 +184,39 @@ __global__ __launch_bounds__(256) void k_batch_cost(const CostTask *__restrict__
     }
 }
 
+__device__ __forceinline__ void batch_path_item(const BatchPathTask *__restrict__ tasks, uint32_t code)
+{
+    const BatchPathTask tk = tasks[code >> 16];
+    const int grp = (int)(code & 0xFFFFu);
+    switch (tk.variant) {
+    case kVarPlain32: path_group<2, true, false, false>(tk, tk.dir, grp); break;
+    case kVarPlain64: path_group<4, true, false, false>(tk, tk.dir, grp); break;
+    case kVarPlain128: path_group<8, true, false, false>(tk, tk.dir, grp); break;
+    case kVarMs32: path_group<2, false, true, false>(tk, tk.dir, grp); break;
+    case kVarMs64: path_group<4, false, true, false>(tk, tk.dir, grp); break;
+    case kVarMs128: path_group<8, false, true, false>(tk, tk.dir, grp); break;
+    case kVarMsDec32: path_group<2, false, true, true>(tk, tk.dir, grp); break;
+    case kVarMsDec64: path_group<4, false, true, true>(tk, tk.dir, grp); break;
+    default: path_group<8, false, true, true>(tk, tk.dir, grp); break;
+    }
+}
+
+// Scanline groupings of the path work list the autotuner picks between:
+//   global   one queue, items taken in queue order by whichever wave is free;
+//   per-XCD  the queue cut into kXcdSlices contiguous slices, one per XCD:
+//            a wave takes items from its own XCD's slice, then from the
+//            others in turn, so neighbouring line groups (whose disparity
+//            windows overlap) are mostly processed under one L2.
+constexpr int kXcdSlices = 8;
+constexpr int kCounterStride = 16; // ints: one 64-byte line per counter
+constexpr int kCounters = kXcdSlices + 1;
+
+__global__ __launch_bounds__(64) void k_batch_reset(int *counter)
+{
+    if (threadIdx.x < kCounters)
+        counter[threadIdx.x * kCounterStride] = 0;
+}
+
 // One wave per block; items are (task << 16 | line group) in queue order.
 __global__ __launch_bounds__(64) void k_batch_path(const BatchPathTask *__restrict__ tasks,
                                                    const uint32_t *__restrict__ items, int nitems,
@@ -192,19 +229,27 @@ __global__ __launch_bounds__(64) void k_batch_path(const BatchPathTask *__restri
         it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
         if (it >= nitems)
             break;
-        const uint32_t code = items[it];
-        const BatchPathTask tk = tasks[code >> 16];
-        const int grp = (int)(code & 0xFFFFu);
-        switch (tk.variant) {
-        case kVarPlain32: path_group<2, true, false, false>(tk, tk.dir, grp); break;
-        case kVarPlain64: path_group<4, true, false, false>(tk, tk.dir, grp); break;
-        case kVarPlain128: path_group<8, true, false, false>(tk, tk.dir, grp); break;
-        case kVarMs32: path_group<2, false, true, false>(tk, tk.dir, grp); break;
-        case kVarMs64: path_group<4, false, true, false>(tk, tk.dir, grp); break;
-        case kVarMs128: path_group<8, false, true, false>(tk, tk.dir, grp); break;
-        case kVarMsDec32: path_group<2, false, true, true>(tk, tk.dir, grp); break;
-        case kVarMsDec64: path_group<4, false, true, true>(tk, tk.dir, grp); break;
-        default: path_group<8, false, true, true>(tk, tk.dir, grp); break;
+        batch_path_item(tasks, items[it]);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_batch_path_xcd(const BatchPathTask *__restrict__ tasks,
+                                                       const uint32_t *__restrict__ items, int nitems,
+                                                       int *__restrict__ counter)
+{
+    // hwreg(HW_REG_XCC_ID, 0, 4): the XCD this wave runs on
+    const int xcd = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & (kXcdSlices - 1));
+    for (int v = 0; v < kXcdSlices; v++) {
+        const int q = (xcd + v) & (kXcdSlices - 1);
+        const int lo = (int)((long)nitems * q / kXcdSlices), hi = (int)((long)nitems * (q + 1) / kXcdSlices);
+        for (;;) {
+            int it = 0;
+            if (threadIdx.x == 0)
+                it = atomicAdd(counter + (1 + q) * kCounterStride, 1);
+            it = lo + __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+            if (it >= hi)
+                break;
+            batch_path_item(tasks, items[it]);
         }
     }
 }
@@ -245,7 +290,9 @@ struct DevArray {
 
 struct Schedule {
     int path_waves_per_simd = 2;
-    int order = 0; // 0: longest lines first; 1: pair by pair, directions in order
+    int order = 0;        // 0: longest lines first; 1: pair by pair, directions in order
+    int census_shape = 0; // census tile kCenShapeW[i] x kCenShapeH[i]
+    int xcd_slices = 0;   // 0: one global path queue; 1: per-XCD slices
 };
 
 } // namespace
@@ -253,13 +300,13 @@ struct Schedule {
 // One lane: a subset of the pairs and their phase work lists, run as one
 // chain of phase kernels on its own stream.
 struct BatchLane {
-    DevArray<PreTask> pre[3];
+    DevArray<PreTask> pre[kCenShapes][3]; // per census tile shape
     DevArray<CostTask> cost[2];
     DevArray<BatchPathTask> path;
     std::vector<uint32_t> items_lpt, items_pair;
     uint32_t *d_items = nullptr; // both orders, back to back
     DevArray<WtaTask> wta;
-    int pre_n[3] = {0, 0, 0}, cost_n[2] = {0, 0}, wta_n = 0; // items per launch
+    int pre_n[kCenShapes][3] = {}, cost_n[2] = {0, 0}, wta_n = 0; // items per launch
     int *d_counter = nullptr;
     bool empty() const { return path.host.empty(); }
 };
@@ -344,10 +391,17 @@ long group_work(const BatchPathTask &t, int grp)
 int enqueue_lane(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc)
 {
     const int pre_blocks = b->cus * 8;
+    const int sh = sc.census_shape;
     for (int k = 0; k < 3; k++)
-        if (ln.pre_n[k])
-            hipLaunchKernelGGL(k_batch_pre, dim3(std::min(ln.pre_n[k], pre_blocks)), dim3(256), 0, s,
-                               ln.pre[k].dev, (int)ln.pre[k].host.size(), ln.pre_n[k]);
+        if (ln.pre_n[sh][k]) {
+            const dim3 g(std::min(ln.pre_n[sh][k], pre_blocks));
+            if (sh == 0)
+                hipLaunchKernelGGL((k_batch_pre<64, 16>), g, dim3(256), 0, s, ln.pre[sh][k].dev,
+                                   (int)ln.pre[sh][k].host.size(), ln.pre_n[sh][k]);
+            else
+                hipLaunchKernelGGL((k_batch_pre<32, 32>), g, dim3(256), 0, s, ln.pre[sh][k].dev,
+                                   (int)ln.pre[sh][k].host.size(), ln.pre_n[sh][k]);
+        }
     for (int k = 0; k < 2; k++)
         if (ln.cost_n[k])
             hipLaunchKernelGGL(k_batch_cost, dim3(std::min(ln.cost_n[k], pre_blocks)), dim3(256), 0, s,
@@ -358,7 +412,10 @@ int enqueue_lane(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc
         const uint32_t *items = ln.d_items + (sc.order == 0 ? 0 : n);
         const int waves = std::min(n, b->cus * 4 * sc.path_waves_per_simd);
         hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, ln.d_counter);
-        hipLaunchKernelGGL(k_batch_path, dim3(waves), dim3(64), 0, s, ln.path.dev, items, n, ln.d_counter);
+        if (sc.xcd_slices)
+            hipLaunchKernelGGL(k_batch_path_xcd, dim3(waves), dim3(64), 0, s, ln.path.dev, items, n, ln.d_counter);
+        else
+            hipLaunchKernelGGL(k_batch_path, dim3(waves), dim3(64), 0, s, ln.path.dev, items, n, ln.d_counter);
     }
     if (ln.wta_n)
         hipLaunchKernelGGL(k_batch_wta, dim3(std::min((ln.wta_n + 3) / 4, b->cus * 8)), dim3(256), 0, s,
@@ -412,15 +469,55 @@ int capture(sgmh_batch *b)
     return 0;
 }
 
-// Time each candidate schedule (median of 3 runs after one warm-up) and keep
-// the fastest.
+// Time candidate schedules (median of 3 runs after one warm-up) and keep the
+// fastest, in two stages: every lane partition x path item order x path waves
+// per SIMD with the default census tile and one global path queue; then, on
+// the best of those, the other census tile shape and the per-XCD scanline
+// grouping, alone and together.
 int autotune(sgmh_batch *b)
 {
     struct Cand {
         int set;
         Schedule sc;
     };
-    std::vector<Cand> cands;
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    hipStream_t s = b->cap;
+    double best = 1e30;
+    std::string js = "[";
+    int ntimed = 0;
+    auto time_cands = [&](const std::vector<Cand> &cands) -> int {
+        for (const Cand &c : cands) {
+            float t[3];
+            if (enqueue_phases(b, s, c.set, c.sc))
+                return -1;
+            for (int r = 0; r < 3; r++) {
+                CK(hipEventRecord(e0, s));
+                if (enqueue_phases(b, s, c.set, c.sc))
+                    return -1;
+                CK(hipEventRecord(e1, s));
+                CK(hipEventSynchronize(e1));
+                CK(hipEventElapsedTime(&t[r], e0, e1));
+            }
+            std::sort(t, t + 3);
+            char buf[256];
+            snprintf(buf, sizeof buf,
+                     "%s{\"lanes\":\"%s\",\"order\":\"%s\",\"path_waves_per_simd\":%d,\"census_tile\":\"%dx%d\","
+                     "\"scanline_grouping\":\"%s\",\"ms\":%.4f}",
+                     ntimed++ ? "," : "", b->sets[c.set].name.c_str(), c.sc.order == 0 ? "longest-first" : "pair-major",
+                     c.sc.path_waves_per_simd, kCenShapeW[c.sc.census_shape], kCenShapeH[c.sc.census_shape],
+                     c.sc.xcd_slices ? "per-XCD" : "global", t[1]);
+            js += buf;
+            if (t[1] < best) {
+                best = t[1];
+                b->set = c.set;
+                b->sched = c.sc;
+            }
+        }
+        return 0;
+    };
+    std::vector<Cand> stage1;
     for (int set = 0; set < (int)b->sets.size(); set++)
         for (int order = 0; order < 2; order++)
             for (int pw : {1, 2, 4}) {
@@ -428,38 +525,22 @@ int autotune(sgmh_batch *b)
                 c.set = set;
                 c.sc.order = order;
                 c.sc.path_waves_per_simd = pw;
-                cands.push_back(c);
+                stage1.push_back(c);
             }
-    hipEvent_t e0, e1;
-    CK(hipEventCreate(&e0));
-    CK(hipEventCreate(&e1));
-    hipStream_t s = b->cap;
-    double best = 1e30;
-    std::string js = "[";
-    for (size_t c = 0; c < cands.size(); c++) {
-        float t[3];
-        if (enqueue_phases(b, s, cands[c].set, cands[c].sc))
-            return -1;
-        for (int r = 0; r < 3; r++) {
-            CK(hipEventRecord(e0, s));
-            if (enqueue_phases(b, s, cands[c].set, cands[c].sc))
-                return -1;
-            CK(hipEventRecord(e1, s));
-            CK(hipEventSynchronize(e1));
-            CK(hipEventElapsedTime(&t[r], e0, e1));
+    if (time_cands(stage1))
+        return -1;
+    std::vector<Cand> stage2;
+    for (int sh = 0; sh < kCenShapes; sh++)
+        for (int xs = 0; xs < 2; xs++) {
+            if (sh == b->sched.census_shape && xs == b->sched.xcd_slices)
+                continue;
+            Cand c{b->set, b->sched};
+            c.sc.census_shape = sh;
+            c.sc.xcd_slices = xs;
+            stage2.push_back(c);
         }
-        std::sort(t, t + 3);
-        char buf[200];
-        snprintf(buf, sizeof buf, "%s{\"lanes\":\"%s\",\"order\":\"%s\",\"path_waves_per_simd\":%d,\"ms\":%.4f}",
-                 c ? "," : "", b->sets[cands[c].set].name.c_str(),
-                 cands[c].sc.order == 0 ? "longest-first" : "pair-major", cands[c].sc.path_waves_per_simd, t[1]);
-        js += buf;
-        if (t[1] < best) {
-            best = t[1];
-            b->set = cands[c].set;
-            b->sched = cands[c].sc;
-        }
-    }
+    if (time_cands(stage2))
+        return -1;
     js += "]";
     b->tuning = js;
     (void)hipEventDestroy(e0);
@@ -559,28 +640,30 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
         LaneSet ls;
         ls.name = name;
         ls.lanes.resize(nl);
-        int pre_items[kMaxLanes][3] = {}, cost_items[kMaxLanes][2] = {}, wta_items[kMaxLanes] = {};
+        int pre_items[kMaxLanes][kCenShapes][3] = {}, cost_items[kMaxLanes][2] = {}, wta_items[kMaxLanes] = {};
         int li = 0;
         auto add_pre = [&](int level, int kind, const uint8_t *src, void *dst, int W, int H, int ostride) {
-            PreTask t;
-            t.src = src;
-            t.dst = dst;
-            t.W = W;
-            t.H = H;
-            t.kind = kind;
-            t.ostride = ostride;
-            int ty;
-            if (kind == PRE_CENSUS) {
-                t.tiles_x = (W + kCenTX - 1) / kCenTX;
-                ty = (H + kCenTY - 1) / kCenTY;
-            } else {
-                const int ow = kind == PRE_HALF ? W / 2 : W, oh = kind == PRE_HALF ? H / 2 : H;
-                t.tiles_x = (ow + 63) / 64;
-                ty = (oh + 3) / 4;
+            for (int sh = 0; sh < kCenShapes; sh++) {
+                PreTask t;
+                t.src = src;
+                t.dst = dst;
+                t.W = W;
+                t.H = H;
+                t.kind = kind;
+                t.ostride = ostride;
+                int ty;
+                if (kind == PRE_CENSUS) {
+                    t.tiles_x = (W + kCenShapeW[sh] - 1) / kCenShapeW[sh];
+                    ty = (H + kCenShapeH[sh] - 1) / kCenShapeH[sh];
+                } else {
+                    const int ow = kind == PRE_HALF ? W / 2 : W, oh = kind == PRE_HALF ? H / 2 : H;
+                    t.tiles_x = (ow + 63) / 64;
+                    ty = (oh + 3) / 4;
+                }
+                t.first_item = pre_items[li][sh][level];
+                pre_items[li][sh][level] += t.tiles_x * ty;
+                ls.lanes[li].pre[sh][level].host.push_back(t);
             }
-            t.first_item = pre_items[li][level];
-            pre_items[li][level] += t.tiles_x * ty;
-            ls.lanes[li].pre[level].host.push_back(t);
         };
         auto add_cost = [&](int level, const uint64_t *cl, const uint64_t *cr, const uint8_t *merge, uint8_t *C,
                             int W, int H, int D) {
@@ -672,8 +755,9 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
         for (size_t q = 0; q < kept.size(); q++) {
             BatchLane &ln = kept[q];
             const int L = kept_idx[q];
-            for (int k = 0; k < 3; k++)
-                ln.pre_n[k] = pre_items[L][k];
+            for (int sh = 0; sh < kCenShapes; sh++)
+                for (int k = 0; k < 3; k++)
+                    ln.pre_n[sh][k] = pre_items[L][sh][k];
             for (int k = 0; k < 2; k++)
                 ln.cost_n[k] = cost_items[L][k];
             ln.wta_n = wta_items[L];
@@ -733,8 +817,9 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
     }
     for (LaneSet &ls : b->sets)
         for (BatchLane &ln : ls.lanes) {
-            for (int k = 0; k < 3 && !rc; k++)
-                rc = upload(ln.pre[k]);
+            for (int sh = 0; sh < kCenShapes; sh++)
+                for (int k = 0; k < 3 && !rc; k++)
+                    rc = upload(ln.pre[sh][k]);
             for (int k = 0; k < 2 && !rc; k++)
                 rc = upload(ln.cost[k]);
             if (!rc)
@@ -750,7 +835,7 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
                     rc = -1;
                 }
             }
-            if (!rc && hipMalloc(&ln.d_counter, 64) != hipSuccess) {
+            if (!rc && hipMalloc(&ln.d_counter, sizeof(int) * kCounterStride * kCounters) != hipSuccess) {
                 set_err("sgmh_batch_plan: cannot allocate the work counter");
                 rc = -1;
             }
@@ -766,8 +851,12 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
             set_err("sgmh_batch_plan: cannot create streams / events");
             rc = -1;
         }
-    if (!rc && (flags & SGMH_BATCH_AUTOTUNE))
+    if (!rc && (flags & SGMH_BATCH_AUTOTUNE)) {
         rc = autotune(b);
+    } else {
+        b->sched.census_shape = (flags & SGMH_BATCH_CENSUS_32X32) ? 1 : 0;
+        b->sched.xcd_slices = (flags & SGMH_BATCH_XCD_SLICES) ? 1 : 0;
+    }
     if (!rc && !(flags & SGMH_BATCH_NO_GRAPH))
         rc = capture(b);
     if (!rc && hipStreamSynchronize(b->cap) != hipSuccess) {
@@ -825,10 +914,12 @@ int sgmh_batch_describe(const sgmh_batch *b, char *buf, size_t len)
         buf, len,
         "{\"pairs\":%zu,\"plain\":%d,\"multiscale\":%d,\"workspace_bytes\":%zu,\"path_items\":%zu,"
         "\"path_tasks\":%zu,\"wta_tasks\":%zu,\"graph\":%s,\"schedule\":{\"lanes\":\"%s\",\"lane_count\":%zu,"
-        "\"order\":\"%s\",\"path_waves_per_simd\":%d},\"tuning\":%s}",
+        "\"order\":\"%s\",\"path_waves_per_simd\":%d,\"census_tile\":\"%dx%d\",\"scanline_grouping\":\"%s\"},"
+        "\"tuning\":%s}",
         b->pairs.size(), plain, ms, b->arena_bytes, items, tasks, wtas, b->graph_ok ? "true" : "false",
         ls.name.c_str(), ls.lanes.size(), b->sched.order == 0 ? "longest-first" : "pair-major",
-        b->sched.path_waves_per_simd, b->tuning.empty() ? "null" : b->tuning.c_str());
+        b->sched.path_waves_per_simd, kCenShapeW[b->sched.census_shape], kCenShapeH[b->sched.census_shape],
+        b->sched.xcd_slices ? "per-XCD" : "global", b->tuning.empty() ? "null" : b->tuning.c_str());
     return n;
 }
 
@@ -844,9 +935,10 @@ void sgmh_batch_destroy(sgmh_batch *b)
         (void)hipGraphDestroy(b->graph);
     for (LaneSet &ls : b->sets)
         for (BatchLane &ln : ls.lanes) {
-        for (auto &a : ln.pre)
-            if (a.dev)
-                (void)hipFree(a.dev);
+        for (auto &shape : ln.pre)
+            for (auto &a : shape)
+                if (a.dev)
+                    (void)hipFree(a.dev);
         for (auto &a : ln.cost)
             if (a.dev)
                 (void)hipFree(a.dev);

@@ -28,12 +28,24 @@ static const int kDirsMs[8][3] = {{1, 0, 40}, {-1, 0, 40}, {0, 1, 20}, {0, -1, 2
                                   {1, 1, 10}, {-1, -1, 10}, {1, -1, 10}, {-1, 1, 10}};
 
 // ---------------------------------------------------------------------------
-// census: 64x16 output tile per 256-thread block, each thread 4 consecutive x
+// census: a TX x TY output tile per 256-thread block, each thread 4 consecutive x
+// (TX * TY = 1024; 64x16 by default, 32x32 the batch's other tile shape)
 // ---------------------------------------------------------------------------
+template <int TX, int TY>
+struct CenTile {
+    static_assert(TX * TY == 1024 && TX % 4 == 0, "a tile is 256 threads x 4 pixels");
+    static constexpr int RowBytes = TX + 8; // x0-4 .. x0+TX+3
+    static constexpr int Rows = TY + 6;     // y0-3 .. y0+TY+2
+    static constexpr int RowWords = RowBytes / 2; // the tile holds each pixel as a uint16
+    static constexpr int Words = Rows * RowWords;
+};
 constexpr int kCenTX = 64, kCenTY = 16;
-constexpr int kCenRowBytes = kCenTX + 8;         // 72: x0-4 .. x0+67
-constexpr int kCenRows = kCenTY + 6;             // 22: y0-3 .. y0+18
-constexpr int kCenRowWords = kCenRowBytes / 2;   // 36: the tile holds each pixel as a uint16
+constexpr int kCenRowBytes = CenTile<64, 16>::RowBytes;
+constexpr int kCenRows = CenTile<64, 16>::Rows;
+constexpr int kCenRowWords = CenTile<64, 16>::RowWords;
+// LDS words a census tile of either shape needs
+constexpr int kCenTileWords = CenTile<64, 16>::Words > CenTile<32, 32>::Words ? CenTile<64, 16>::Words
+                                                                              : CenTile<32, 32>::Words;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -60,9 +72,9 @@ __device__ __forceinline__ uint32_t and_or(uint32_t a, uint32_t m, uint32_t b)
     return r;
 }
 
-// One 64x16 output tile by 256 threads (all of them must call it): the
-// (64+8)x(16+6) input window is staged in `tile` (kCenRows*kCenRowWords words
-// of LDS, 8-byte aligned, one uint16 per pixel) with clamped coordinates, then
+// One TX x TY output tile by 256 threads (all of them must call it): the
+// (TX+8)x(TY+6) input window is staged in `tile` (CenTile::Words words of
+// LDS, 8-byte aligned, one uint16 per pixel) with clamped coordinates, then
 // each thread builds the codes of 4 consecutive pixels from 7 rows x 6 dwords
 // of LDS.
 //
@@ -73,6 +85,7 @@ __device__ __forceinline__ uint32_t and_or(uint32_t a, uint32_t m, uint32_t b)
 // is taken into bit 8 + n % 8 of word n / 8 by one v_and_or_b32, so word k
 // ends up holding code byte k of the pair's first pixel in its byte 1 and of
 // the second pixel in its byte 3; byte permutes then assemble the codes.
+template <int TX = 64, int TY = 16>
 __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uint64_t *__restrict__ out,
                                             int W, int H, int ostride, int x0, int y0,
                                             uint32_t *__restrict__ tile)
@@ -83,7 +96,9 @@ __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uin
     // columns x0-4 .. x0+67 all lie inside a row, of a width that is a multiple
     // of 4 (and an image base aligned to 4 bytes) is staged by aligned 4-byte
     // words; the others by clamped bytes.
-    if (x0 >= 4 && x0 + kCenTX + 4 <= W && (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0) {
+    using T = CenTile<TX, TY>;
+    constexpr int kCenRows = T::Rows, kCenRowBytes = T::RowBytes, kCenRowWords = T::RowWords;
+    if (x0 >= 4 && x0 + TX + 4 <= W && (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0) {
         constexpr int kWords = kCenRows * (kCenRowBytes / 4), kIters = (kWords + 255) / 256;
         uint32_t wv[kIters];
 #pragma unroll
@@ -125,7 +140,7 @@ __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uin
         }
     }
     __syncthreads();
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int tx = threadIdx.x & (TX / 4 - 1), ty = threadIdx.x / (TX / 4);
     const int x = x0 + 4 * tx, y = y0 + ty;
     if (y >= H || x >= W)
         return;

@@ -85,7 +85,7 @@ class Pair(ctypes.Structure):
 
 
 PLAIN, MULTISCALE = 0, 1
-BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
+BATCH_AUTOTUNE, BATCH_NO_GRAPH, BATCH_CENSUS_32X32, BATCH_XCD_SLICES = 1, 2, 4, 8
 
 
 def hip_lib() -> ctypes.CDLL:

@@ -40,7 +40,10 @@ SMALL = [
 ]
 
 
-@pytest.mark.parametrize("flags", [sgm_amd.BATCH_AUTOTUNE, sgm_amd.BATCH_NO_GRAPH, 0])
+@pytest.mark.parametrize("flags", [sgm_amd.BATCH_AUTOTUNE, sgm_amd.BATCH_NO_GRAPH, 0,
+                                   # the schedule choices the autotuner may make, pinned
+                                   sgm_amd.BATCH_CENSUS_32X32, sgm_amd.BATCH_XCD_SLICES,
+                                   sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES | sgm_amd.BATCH_NO_GRAPH])
 def test_mixed_batch_matches_oracle(ctx, flags):
     data = []
     for sp in SMALL:
@@ -67,16 +70,26 @@ def test_mixed_batch_matches_oracle(ctx, flags):
         assert np.array_equal(d_out.cpu().numpy(), w), f"loop: {sp}"
     info = b.describe()
     assert info["pairs"] == len(SMALL)
+    if not flags & sgm_amd.BATCH_AUTOTUNE:
+        sched = info["schedule"]
+        assert sched["census_tile"] == ("32x32" if flags & sgm_amd.BATCH_CENSUS_32X32 else "64x16")
+        assert sched["scanline_grouping"] == ("per-XCD" if flags & sgm_amd.BATCH_XCD_SLICES else "global")
+    else:
+        # the autotuner timed both census tile shapes and both groupings
+        tried = {(t["census_tile"], t["scanline_grouping"]) for t in info["tuning"]}
+        assert tried == {(c, g) for c in ("64x16", "32x32") for g in ("global", "per-XCD")}
     b.close()
 
 
-def test_full_batch_goldens(ctx, goldens):
+@pytest.mark.parametrize("flags", [sgm_amd.BATCH_AUTOTUNE,
+                                   sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES])
+def test_full_batch_goldens(ctx, goldens, flags):
     data, specs = [], sgm_amd.batch_spec()
     for sp in specs:
         L, R = sgm_amd.make_scene(sp["scene"])
         data.append((sp, L, R))
     pairs, keep = upload(data)
-    b = ctx.batch(pairs, sgm_amd.BATCH_AUTOTUNE)
+    b = ctx.batch(pairs, flags)
     b.run(torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     gold = {g["name"]: g["fnv"] for g in goldens["batch"]}
