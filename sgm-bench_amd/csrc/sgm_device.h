@@ -907,6 +907,7 @@ __device__ __forceinline__ void wta_batch(const uint8_t *__restrict__ L, size_t 
 // offsets.  Lane groups past the last line recompute the last line and store
 // the same bytes to the same places.
 constexpr uint32_t kWinAbsent = 255u;
+constexpr int kWinMaxTiles = 1024;      // tiles along a line (W, H <= 16384)
 constexpr uint32_t kWinPenalty = 1000u; // cost of an absent disparity inside a step: L > 255
 
 struct WinArgs {
@@ -1033,11 +1034,14 @@ __device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask
     // steps q in [0, 16*nt) on a tile-aligned axis: x = q (DIR > 0) or 16*nt-1-q
     const int q0 = DIR > 0 ? 0 : 16 * nt - W, qend = DIR > 0 ? W : 16 * nt;
     const int rowc = ln * W + kLead, rowp = ln * W; // code / pixel index of (0, ln)
-    auto seg_raw = [&](int k) -> uint32_t {
-        const int kk = k < nt ? k : nt - 1;
-        const int tx = DIR > 0 ? kk : nt - 1 - kk;
-        return __builtin_amdgcn_readfirstlane((uint32_t)a.blk[tile_y * a.tiles_x + tx]);
-    };
+    // The line's tile ranges in traversal order, staged in LDS once: a segment
+    // reads its range a few segments ahead from there, so no wait for a global
+    // load (which would drain the code prefetch ring) falls inside the line.
+    __shared__ uint8_t segs[kWinMaxTiles];
+    for (int k = lane; k < nt; k += 64)
+        segs[k] = a.blk[tile_y * a.tiles_x + (DIR > 0 ? k : nt - 1 - k)];
+    __syncthreads(); // one wave per workgroup: orders the staging before the reads
+    auto seg_raw = [&](int k) -> uint32_t { return segs[k < nt ? k : nt - 1]; }; // the same in every lane
     auto half_code = [](const v4u &v, int half) -> uint64_t {
         return half ? ((uint64_t)v[3] << 32) | v[2] : ((uint64_t)v[1] << 32) | v[0];
     };
@@ -1086,7 +1090,8 @@ __device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask
 
     WinState<NB> S;
     S.reset();
-    WinSeg cur = win_seg<NB>(seg_raw(0)), nxt = win_seg<NB>(seg_raw(1)), prev = cur;
+    WinSeg cur = win_seg<NB>(__builtin_amdgcn_readfirstlane(seg_raw(0))),
+           nxt = win_seg<NB>(__builtin_amdgcn_readfirstlane(seg_raw(1))), prev = cur;
     uint32_t raw_nn = seg_raw(2);
     // prologue: the loads of segment 0's first PF steps, as the last PF steps
     // of a segment issue them (a dropped store stands for each step's store)
@@ -1197,7 +1202,7 @@ __device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask
         }
         prev = cur;
         cur = nxt;
-        nxt = win_seg<NB>(raw_nn);
+        nxt = win_seg<NB>(__builtin_amdgcn_readfirstlane(raw_nn));
         raw_nn = seg_raw(k + 3);
     }
 }
@@ -1231,11 +1236,12 @@ __device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask
     const rsrc_t rn = make_rsrc(narrow, (uint32_t)(W * H) * 32u);
     const uint32_t P1 = (uint32_t)t.P1, P2 = kPlainP2, inval = kPlainInvalid;
     const int q0 = dir > 0 ? 0 : 16 * nt - H, qend = dir > 0 ? H : 16 * nt;
-    auto seg_raw = [&](int k) -> uint32_t {
-        const int kk = k < nt ? k : nt - 1;
-        const int ty = dir > 0 ? kk : nt - 1 - kk;
-        return __builtin_amdgcn_readfirstlane((uint32_t)a.blk[ty * a.tiles_x + tile_x]);
-    };
+    // the line's tile ranges in traversal order, staged in LDS once (as for rows)
+    __shared__ uint8_t segs[kWinMaxTiles];
+    for (int k = lane; k < nt; k += 64)
+        segs[k] = a.blk[(dir > 0 ? k : nt - 1 - k) * a.tiles_x + tile_x];
+    __syncthreads(); // one wave per workgroup: orders the staging before the reads
+    auto seg_raw = [&](int k) -> uint32_t { return segs[k < nt ? k : nt - 1]; }; // the same in every lane
     auto half_code = [](const v4u &v, int half) -> uint64_t {
         return half ? ((uint64_t)v[3] << 32) | v[2] : ((uint64_t)v[1] << 32) | v[0];
     };
@@ -1265,7 +1271,8 @@ __device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask
 
     WinState<NB> S;
     S.reset();
-    WinSeg cur = win_seg<NB>(seg_raw(0)), nxt = win_seg<NB>(seg_raw(1)), prev = cur;
+    WinSeg cur = win_seg<NB>(__builtin_amdgcn_readfirstlane(seg_raw(0))),
+           nxt = win_seg<NB>(__builtin_amdgcn_readfirstlane(seg_raw(1))), prev = cur;
     uint32_t raw_nn = seg_raw(2);
 #pragma unroll
     for (int k = 0; k < PF; k++) {
@@ -1367,7 +1374,7 @@ __device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask
         }
         prev = cur;
         cur = nxt;
-        nxt = win_seg<NB>(raw_nn);
+        nxt = win_seg<NB>(__builtin_amdgcn_readfirstlane(raw_nn));
         raw_nn = seg_raw(k + 3);
     }
 }
