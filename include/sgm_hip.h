@@ -70,6 +70,20 @@ int sgmh_set_timing(sgmh_ctx *ctx, int on);
 /* Wait for the last call and read its phase times. */
 int sgmh_get_times(sgmh_ctx *ctx, sgmh_times *t);
 
+/* No counterpart in the reference (the reference's kernels have one layout):
+ * how this context's plain path kernel lays scanlines on a wave64.  Every
+ * layout gives the same output.
+ *   SGMH_LAYOUT_AUTO   (default) picks by frame size and D;
+ *   SGMH_LAYOUT_LINES4 four scanlines per wave, 16 lanes each, D/16
+ *                      disparities per lane;
+ *   SGMH_LAYOUT_WAVE   one scanline per wave, D/64 disparities per lane
+ *                      (D = 64, 128 or 256; other D fail at the call).
+ * Returns -1 for an unknown layout. */
+#define SGMH_LAYOUT_AUTO 0
+#define SGMH_LAYOUT_LINES4 1
+#define SGMH_LAYOUT_WAVE 2
+int sgmh_set_path_layout(sgmh_ctx *ctx, int layout);
+
 /* ---- plain SGM: replaces the body of sgm_run_fn (common/sgm.h:29-31) ---- */
 
 /* Host buffers, synchronous: disp is complete on return (common/sgm.h:27-31,

@@ -465,16 +465,21 @@ __device__ __forceinline__ LineGeom line_geom(int i, int dx, int dy, int W, int 
 // 64-bit row shift by one lane; the lane without a source (lane 0 for shr,
 // 15 for shl) reads 0 and takes `in` instead, which is 0 in every other lane:
 // the OR folds into the DPP move (v_or_b32_dpp, bound_ctrl:1), one op per half.
+// (LPL = 64: the same across the whole wave, lanes 0 / 63 without a source)
+template <int LPL = kRowLanes>
 __device__ __forceinline__ uint64_t dpp64_shr1(uint64_t in, uint64_t v)
 {
-    const uint32_t lo = dpp<kRowShr1>((uint32_t)v) | (uint32_t)in;
-    const uint32_t hi = dpp<kRowShr1>((uint32_t)(v >> 32)) | (uint32_t)(in >> 32);
+    constexpr int ctl = LineLanes<LPL>::shr;
+    const uint32_t lo = dpp<ctl>((uint32_t)v) | (uint32_t)in;
+    const uint32_t hi = dpp<ctl>((uint32_t)(v >> 32)) | (uint32_t)(in >> 32);
     return ((uint64_t)hi << 32) | lo;
 }
+template <int LPL = kRowLanes>
 __device__ __forceinline__ uint64_t dpp64_shl1(uint64_t in, uint64_t v)
 {
-    const uint32_t lo = dpp<kRowShl1>((uint32_t)v) | (uint32_t)in;
-    const uint32_t hi = dpp<kRowShl1>((uint32_t)(v >> 32)) | (uint32_t)(in >> 32);
+    constexpr int ctl = LineLanes<LPL>::shl;
+    const uint32_t lo = dpp<ctl>((uint32_t)v) | (uint32_t)in;
+    const uint32_t hi = dpp<ctl>((uint32_t)(v >> 32)) | (uint32_t)(in >> 32);
     return ((uint64_t)hi << 32) | lo;
 }
 
@@ -498,10 +503,13 @@ constexpr uint32_t kFar = 0x80000000u;
 // 3-4 % faster path kernels at D = 64 where there is at least one row wave
 // per CU (1280x720, 1920x1080), 6 % slower where rows are few (640x480) and
 // 1 % slower at D = 128, so the launch picks it by frame height and D.
-template <int DPL, bool SAT, int DX, bool BCAST, class A>
+// LPL: lanes per scanline (16, or 64 for the wave-line layout, where lane l
+// owns disparities l*DPL .. with DPL = D/64 and the shift crosses the wave).
+template <int DPL, bool SAT, int DX, bool BCAST, class A, int LPL = kRowLanes>
 __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, LineGeom ln,
                                                 int l, int maxsteps)
 {
+    static_assert(!BCAST || LPL == kRowLanes, "the left-code broadcast is per DPP row");
     constexpr int PF = DPL > 8 ? DPL : 8;
     // Rows are the longest scanlines of the frame and set its critical path:
     // give their waves issue priority over the column / diagonal waves that
@@ -528,7 +536,7 @@ __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, Li
     // prefetched cl[x_s] and the window code entering at step s; only the
     // lane that takes the entering code loads it, the others read 0
     const int enter = DX > 0 ? 0 : -(D - 1);
-    const uint32_t nbias = l == (DX > 0 ? 0 : kRowLanes - 1) ? 0u : kFar;
+    const uint32_t nbias = l == (DX > 0 ? 0 : LPL - 1) ? 0u : kFar;
     const uint32_t lbias = BCAST && l != 0 ? kFar : 0u; // BCAST: lane 0 alone loads the left code
     uint64_t pl[PF], pn[PF];
 #pragma unroll
@@ -548,34 +556,39 @@ __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, Li
     // One block of PF steps.  CHECK: some step of the block may have x-d < 0
     // for a lane's disparities, or run past the end of the line.  Blocks
     // without either (all but ~D/PF + 1 of them) skip both selects.
+    // The costs of step k of a block, at x = xk: shift the window to that
+    // step, then the Hamming costs (CHECK: with the invalid select).
+    auto costs = [&](int k, int xk, auto check_tag, uint32_t (&C)[DPL]) {
+        constexpr bool CHECK = decltype(check_tag)::value;
+        if constexpr (DX > 0) {
+            const int slot = ((-k) % DPL + DPL) % DPL;        // logical 0 at step k
+            phys[slot] = dpp64_shr1<LPL>(pn[k], phys[slot]); // was logical DPL-1
+        } else {
+            const int slot = ((k - 1) % DPL + DPL) % DPL;     // logical DPL-1 at step k
+            phys[slot] = dpp64_shl1<LPL>(pn[k], phys[slot]); // was logical 0
+        }
+        const uint64_t cL = BCAST ? row_bcast0(pl[k]) : pl[k];
+        const int rel = xk - lbase;
+#pragma unroll
+        for (int j = 0; j < DPL; j++) {
+            const int slot = DX > 0 ? (((j - k) % DPL) + DPL) % DPL : (j + k) % DPL;
+            const uint32_t h = (uint32_t)__popcll(cL ^ phys[slot]);
+            C[j] = CHECK ? (rel >= j ? h : inval) : h;
+        }
+    };
     auto block = [&](int s0, auto check_tag) {
         constexpr bool CHECK = decltype(check_tag)::value;
 #pragma unroll
         for (int k = 0; k < PF; k++) {
             const int s = s0 + k;
-            // shift the window to step s
-            if constexpr (DX > 0) {
-                const int slot = ((-k) % DPL + DPL) % DPL;        // logical 0 at step s
-                phys[slot] = dpp64_shr1(pn[k], phys[slot]);      // was logical DPL-1
-            } else {
-                const int slot = ((k - 1) % DPL + DPL) % DPL;     // logical DPL-1 at step s
-                phys[slot] = dpp64_shl1(pn[k], phys[slot]);      // was logical 0
-            }
-            const uint64_t cL = BCAST ? row_bcast0(pl[k]) : pl[k];
             uint32_t C[DPL];
-            const int rel = x - lbase;
-#pragma unroll
-            for (int j = 0; j < DPL; j++) {
-                const int slot = DX > 0 ? (((j - k) % DPL) + DPL) % DPL : (j + k) % DPL;
-                const uint32_t h = (uint32_t)__popcll(cL ^ phys[slot]);
-                C[j] = CHECK ? (rel >= j ? h : inval) : h;
-            }
+            costs(k, x, check_tag, C);
             // ring slot k is dead now: refill it for step s + PF in place
             __builtin_amdgcn_sched_barrier(0);
             const int xp = x + PF * DX;
             pl[k] = ld64(rl, (uint32_t)(row + xp) * 8u + lbias);
             pn[k] = ld64(rr, (uint32_t)(row + xp + enter) * 8u + nbias);
-            sgm_step<DPL, SAT>(L, C, mn, P1, P2);
+            sgm_step<DPL, SAT, LPL>(L, C, mn, P1, P2);
             st_bytes<DPL, (DPL >= 8)>(ro, (!CHECK || s < steps) ? off : kDrop, L);
             off += ostep;
             x += DX;
@@ -610,10 +623,12 @@ __device__ __forceinline__ void path_horizontal(const A &a, const DirTask &t, Li
 // VERT: 0 any direction (cost select per step), 1 a column (x fixed: the
 // invalid disparities are fixed too, folded in by a max), 2 columns with no
 // invalid disparity at all (x >= D-1 on every line of the wave).
-template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, int VERT, class A>
+// LPL: lanes per scanline, as for path_horizontal (64: census-fed only).
+template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, int VERT, class A, int LPL = kRowLanes>
 __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineGeom ln, int l,
                                              int maxsteps)
 {
+    static_assert(LPL == kRowLanes || FROM_CENSUS, "the wave-line layout reads census codes");
     constexpr int PF = DPL <= 4 ? 8 : (DPL <= 8 ? 4 : 2);
     const int W = a.W, D = a.D, dx = t.dx, dy = t.dy;
     const uint32_t P1 = (uint32_t)t.P1, P2 = a.P2, inval = a.invalid;
@@ -684,6 +699,21 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
 
     // One block of PF steps; CHECK: a step of the block may lie past the end
     // of its line (always possible on diagonals, whose lines differ in length).
+    // census-fed costs of ring slot k, at x = xk
+    auto costs = [&](int k, int xk, uint32_t (&C)[DPL]) {
+        const uint64_t cL = rq[k][DPL];
+        const int rel = xk - lbase;
+#pragma unroll
+        for (int j = 0; j < DPL; j++) {
+            const uint32_t h = (uint32_t)__popcll(cL ^ rq[k][j]);
+            if constexpr (VERT == 2)
+                C[j] = h;
+            else if constexpr (VERT == 1)
+                C[j] = h > inv[j] ? h : inv[j];
+            else
+                C[j] = rel >= j ? h : inval;
+        }
+    };
     auto block = [&](int s0, auto check_tag) {
         constexpr bool CHECK = decltype(check_tag)::value;
 #pragma unroll
@@ -691,18 +721,7 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
             const int s = s0 + k;
             uint32_t C[DPL];
             if constexpr (FROM_CENSUS) {
-                const uint64_t cL = rq[k][DPL];
-                const int rel = x - lbase;
-#pragma unroll
-                for (int j = 0; j < DPL; j++) {
-                    const uint32_t h = (uint32_t)__popcll(cL ^ rq[k][j]);
-                    if constexpr (VERT == 2)
-                        C[j] = h;
-                    else if constexpr (VERT == 1)
-                        C[j] = h > inv[j] ? h : inv[j];
-                    else
-                        C[j] = rel >= j ? h : inval;
-                }
+                costs(k, x, C);
             } else if constexpr (!PK) {
 #pragma unroll
                 for (int j = 0; j < DPL; j++)
@@ -722,7 +741,7 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
             if constexpr (PK)
                 sgm_step_sat_pk<NW>(Lw, Cw, mn, P1, P2);
             else
-                sgm_step<DPL, SAT>(L, C, mn, P1, P2);
+                sgm_step<DPL, SAT, LPL>(L, C, mn, P1, P2);
             uint32_t off;
             if constexpr (!DECIM) {
                 off = (uint32_t)pix * (uint32_t)D + (uint32_t)lbase;
@@ -759,17 +778,19 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
         block(s0, std::true_type{});
 }
 
-// Scanlines grp*4 .. grp*4+3 of direction t, one per 16-lane row of the wave.
+// Scanlines grp*4 .. grp*4+3 of direction t, one per 16-lane row of the wave
+// (LPL = 16), or scanline grp alone across the wave (LPL = 64).
 // A is any struct with PathArgs' frame fields (cl, cr, cost, W, H, D, P2,
 // invalid, out_bytes).
-template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, class A, bool BCAST = false>
+template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, class A, bool BCAST = false, int LPL = kRowLanes>
 __device__ __forceinline__ void path_group(const A &a, const DirTask &t, int grp)
 {
-    const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
-    const int base = grp * kLinesPerWave;
+    constexpr int LPW = 64 / LPL; // scanlines per wave
+    const int lane = threadIdx.x & 63, g = lane / LPL, l = lane % LPL;
+    const int base = grp * LPW;
     int maxsteps = 0;
 #pragma unroll
-    for (int q = 0; q < kLinesPerWave; q++)
+    for (int q = 0; q < LPW; q++)
         if (base + q < t.nlines)
             maxsteps = max(maxsteps, line_geom(base + q, t.dx, t.dy, a.W, a.H).steps);
     const int line = base + g;
@@ -779,20 +800,20 @@ __device__ __forceinline__ void path_group(const A &a, const DirTask &t, int grp
     if constexpr (FROM_CENSUS && !DECIM) {
         if (t.dy == 0) {
             if (t.dx > 0)
-                path_horizontal<DPL, SAT, 1, BCAST, A>(a, t, ln, l, maxsteps);
+                path_horizontal<DPL, SAT, 1, BCAST, A, LPL>(a, t, ln, l, maxsteps);
             else
-                path_horizontal<DPL, SAT, -1, BCAST, A>(a, t, ln, l, maxsteps);
+                path_horizontal<DPL, SAT, -1, BCAST, A, LPL>(a, t, ln, l, maxsteps);
             return;
         }
     }
     if (FROM_CENSUS && t.dx == 0) {
         // line i of a column direction is column x = i
         if (base >= a.D - 1)
-            path_generic<DPL, FROM_CENSUS, SAT, DECIM, 2, A>(a, t, ln, l, maxsteps);
+            path_generic<DPL, FROM_CENSUS, SAT, DECIM, 2, A, LPL>(a, t, ln, l, maxsteps);
         else
-            path_generic<DPL, FROM_CENSUS, SAT, DECIM, 1, A>(a, t, ln, l, maxsteps);
+            path_generic<DPL, FROM_CENSUS, SAT, DECIM, 1, A, LPL>(a, t, ln, l, maxsteps);
     } else {
-        path_generic<DPL, FROM_CENSUS, SAT, DECIM, 0, A>(a, t, ln, l, maxsteps);
+        path_generic<DPL, FROM_CENSUS, SAT, DECIM, 0, A, LPL>(a, t, ln, l, maxsteps);
     }
 }
 

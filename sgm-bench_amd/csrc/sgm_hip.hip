@@ -107,20 +107,24 @@ __global__ __launch_bounds__(256) void k_cost(const uint64_t *__restrict__ cl,
 // interleaved (their lengths vary along the range), so they keep block order.
 constexpr int kXcds = 8;
 constexpr int kRowBcastMinH = 512; // 2 x 512 / 4 row waves = 256, one per CU
+constexpr long kWaveLinesMaxPixelsD128 = 1280L * 720; // D = 128 frames up to this size take one line per wave
 
-template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, bool BCAST = false>
+// LPL: lanes per scanline, 16 (four lines per wave) or 64 (the wave-line
+// layout, one line per wave, plain census-fed paths only).
+template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, bool BCAST = false, int LPL = kRowLanes>
 __global__ __launch_bounds__(64) void k_path(PathArgs a)
 {
+    constexpr int LPW = 64 / LPL;
     int k = 0;
     while (k + 1 < a.ndirs && (int)blockIdx.x >= a.dir[k + 1].first_block)
         k++;
     const DirTask t = a.dir[k];
     const int i = (int)blockIdx.x - t.first_block;
-    const int n8 = (t.nlines + kLinesPerWave * kXcds - 1) / (kLinesPerWave * kXcds) * kXcds;
+    const int n8 = (t.nlines + LPW * kXcds - 1) / (LPW * kXcds) * kXcds;
     const int grp = t.dx == 0 ? (i % kXcds) * (n8 / kXcds) + i / kXcds : i;
-    if (grp * kLinesPerWave >= t.nlines)
+    if (grp * LPW >= t.nlines)
         return; // padding block
-    path_group<DPL, FROM_CENSUS, SAT, DECIM, PathArgs, BCAST>(a, t, grp);
+    path_group<DPL, FROM_CENSUS, SAT, DECIM, PathArgs, BCAST, LPL>(a, t, grp);
 }
 
 template <int LPP, int NDIRS>
@@ -206,14 +210,30 @@ static int launch_path_t(hipStream_t s, const PathArgs &a, int blocks)
     return 0;
 }
 
+// the wave-line layout: plain census-fed paths, DPL = D/64
+static int launch_path_wave(hipStream_t s, const PathArgs &a, int blocks)
+{
+    switch (a.D / 64) {
+    case 1: hipLaunchKernelGGL((k_path<1, true, false, false, false, 64>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_path<2, true, false, false, false, 64>), dim3(blocks), dim3(64), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_path<4, true, false, false, false, 64>), dim3(blocks), dim3(64), 0, s, a); break;
+    default: set_err("sgmh: the wave-line layout needs D = 64, 128 or 256 (D = %d)", a.D); return -1;
+    }
+    CK(hipGetLastError());
+    return 0;
+}
+
 // Fill the direction table and launch: census-fed 4-path launches with the
 // row directions last (see below), the others with the longest lines first,
 // so the first blocks the dispatcher places are the longest-running ones.
 // Two variants are instantiated: census-fed plain paths and cost-volume-fed
 // saturating multiscale paths (full or decimated store).
 static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndirs,
-                        bool from_census, bool sat, uint8_t *vol, size_t dir_stride, int decimate)
+                        bool from_census, bool sat, uint8_t *vol, size_t dir_stride, int decimate,
+                        bool wave_lines = false)
 {
+    REQUIRE(!wave_lines || (from_census && !sat && !decimate), "sgmh: the wave-line layout is for plain paths");
+    const int lpw = wave_lines ? 1 : kLinesPerWave; // scanlines per wave
     // every byte offset the kernels form must fit the 32-bit buffer offsets
     const size_t in_bytes = (size_t)a.W * a.H * (from_census ? 8 : (size_t)a.D);
     REQUIRE(dir_stride < (1ull << 31) && in_bytes < (1ull << 31),
@@ -248,9 +268,11 @@ static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndi
         t.first_block = blocks;
         t.L = vol + (size_t)k * dir_stride;
         // line groups of this direction, padded to a multiple of the XCD count
-        const int groups = (t.nlines + kLinesPerWave - 1) / kLinesPerWave;
+        const int groups = (t.nlines + lpw - 1) / lpw;
         blocks += (groups + kXcds - 1) / kXcds * kXcds;
     }
+    if (wave_lines)
+        return launch_path_wave(s, a, blocks);
     // the broadcast row variant pays where row waves are at least one per CU,
     // at D = 64 (measured; at D = 128 it is 1 % slower)
     if (from_census && !sat && !decimate)
@@ -321,10 +343,31 @@ static hipStream_t pick(sgmh_ctx *ctx, void *stream)
     return (hipStream_t)stream;
 }
 
+// The automatic layout choice for plain paths, from a sweep of both layouts
+// over 640x480 .. 1920x1080 and D = 64, 128, 256 on one MI355X (4-path
+// kernel, ms, four lines / one line per wave):
+//   D = 64:  0.076 / 0.097 at 640x480, 0.262 / 0.575 at 1080p -- never;
+//   D = 128: 0.122 / 0.107 at 640x480, 0.316 / 0.300 at 1280x720,
+//            0.473 / 0.664 at 1080p -- up to 1280x720;
+//   D = 256: 0.242 / 0.167 at 640x480, 1.122 / 1.045 at 1080p -- always.
+// Four lines per wave amortise the per-step work (the reduction over D, the
+// loads, the shift) over four lines; one line per wave shortens each line's
+// chain of steps, which pays where the lanes own many disparities (DPL = D/16
+// of 8 or 16) or the frame holds too few lines to fill the GPU.
+static bool wave_lines_pay(int W, int H, int D, int ndirs)
+{
+    if (ndirs > 4)
+        return false;
+    if (D == 256)
+        return true;
+    return D == 128 && (long)W * H <= kWaveLinesMaxPixelsD128;
+}
+
 static int plain_paths(sgmh_ctx *ctx, hipStream_t s, const uint64_t *cl, const uint64_t *cr, int W,
                        int H, int D, const int (*dirs)[3], int ndirs, uint8_t *vol, size_t dir_stride)
 {
-    (void)ctx;
+    const int layout = ctx ? ctx->path_layout : SGMH_LAYOUT_AUTO;
+    const bool wave = layout == SGMH_LAYOUT_WAVE || (layout == SGMH_LAYOUT_AUTO && wave_lines_pay(W, H, D, ndirs));
     PathArgs a{};
     a.cl = cl;
     a.cr = cr;
@@ -333,7 +376,7 @@ static int plain_paths(sgmh_ctx *ctx, hipStream_t s, const uint64_t *cl, const u
     a.D = D;
     a.P2 = kPlainP2;
     a.invalid = kPlainInvalid;
-    return launch_paths(s, a, dirs, ndirs, true, false, vol, dir_stride, 0);
+    return launch_paths(s, a, dirs, ndirs, true, false, vol, dir_stride, 0, wave);
 }
 
 // Census code buffers: (N + kLead) codes.  The kLead slack before pixel 0 is
@@ -648,6 +691,15 @@ int sgmh_set_timing(sgmh_ctx *c, int on)
 {
     REQUIRE(c, "sgmh_set_timing: NULL context");
     c->timing = on != 0;
+    return 0;
+}
+
+int sgmh_set_path_layout(sgmh_ctx *c, int layout)
+{
+    REQUIRE(c, "sgmh_set_path_layout: NULL context");
+    REQUIRE(layout == SGMH_LAYOUT_AUTO || layout == SGMH_LAYOUT_LINES4 || layout == SGMH_LAYOUT_WAVE,
+            "sgmh_set_path_layout: unknown layout %d", layout);
+    c->path_layout = layout;
     return 0;
 }
 

@@ -42,6 +42,7 @@ struct sgmh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool timing = false;
+    int path_layout = 0;                       // SGMH_LAYOUT_*
     std::vector<hipEvent_t> events;
     size_t ev_used = 0;
     std::vector<std::pair<int, std::pair<int, int>>> marks; // phase, ev begin, ev end

@@ -1,14 +1,18 @@
 // sgm_kernels.h -- device-side building blocks shared by the SGM kernels
 // (gfx950, wave64).
 //
-// Work layout used by every aggregation kernel: one scanline per 16-lane DPP
-// row, four scanlines per wave64.  Lane l of a row owns the DPL = D/16
+// Work layout of the aggregation kernels: one scanline per 16-lane DPP row,
+// four scanlines per wave64.  Lane l of a row owns the DPL = D/16
 // consecutive disparities d = l*DPL + j (j = 0..DPL-1) in registers, so
 //   * the d-1 / d+1 neighbours inside a lane are registers,
 //   * the two that cross a lane boundary are one row_shr:1 / row_shl:1 DPP move,
 //   * min over all D is an in-lane min3 chain plus four DPP min steps
 //     (quad_perm x2, row_half_mirror, row_mirror) that leave the row minimum in
 //     every lane of the row: no LDS, no barrier, no readlane.
+// The plain path kernel also has a wave-line layout for frames too small to
+// fill the GPU with four lines per wave: one scanline per wave64, DPL = D/64
+// (one disparity per lane at D = 64), neighbours by wave_shr:1 / wave_shl:1
+// and min over D by a 64-lane DPP reduction read from lane 63.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -60,6 +64,41 @@ __device__ __forceinline__ uint32_t row_min16(uint32_t v)
     return v;
 }
 
+// The wave-line layout: one scanline per wave64, lane l owning disparities
+// l*DPL .. l*DPL+DPL-1 (DPL = D/64).  Its neighbours across lanes move by the
+// whole-wave DPP shifts, and min over D is a reduction over all 64 lanes.
+constexpr int kWaveShr1 = 0x138;   // lane i <- lane i-1 across the wave (lane 0 has none)
+constexpr int kWaveShl1 = 0x130;   // lane i <- lane i+1 across the wave (lane 63 has none)
+constexpr int kRowBcast15 = 0x142; // lane 15 of each row to the whole next row
+constexpr int kRowBcast31 = 0x143; // lane 31 to rows 2 and 3
+
+// Minimum over all 64 lanes, wave-uniform: the row minimum in every lane,
+// then rows 1 and 3 take in rows 0 and 2 (row_bcast:15, rows 0 and 2 masked
+// off) and rows 2 and 3 take in row 1 (row_bcast:31, rows 0 and 1 masked
+// off), which leaves the minimum of all four rows in row 3; lane 63 is read.
+// Each broadcast step is one v_min_u32_dpp whose masked-off rows keep their
+// value (the destination is also the second operand).  Written out, the
+// compiler forms it as a move, a masked DPP move and a min.  The s_nop gives
+// the DPP read the two wait states a VALU result needs before DPP reads it
+// (the compiler's hazard checks do not look inside inline asm).
+__device__ __forceinline__ uint32_t wave_min64(uint32_t v)
+{
+    v = row_min16(v);
+    asm("s_nop 1\n\tv_min_u32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf" : "+v"(v));
+    asm("s_nop 1\n\tv_min_u32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf" : "+v"(v));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Lanes per scanline LPL: 16 (four lines per wave, DPP row moves) or 64 (one
+// line per wave, whole-wave moves).
+template <int LPL>
+struct LineLanes {
+    static_assert(LPL == 16 || LPL == 64, "a scanline spans one DPP row or the whole wave");
+    static constexpr int shr = LPL == 16 ? kRowShr1 : kWaveShr1;
+    static constexpr int shl = LPL == 16 ? kRowShl1 : kWaveShl1;
+    static __device__ __forceinline__ uint32_t min(uint32_t v) { return LPL == 16 ? row_min16(v) : wave_min64(v); }
+};
+
 // One recurrence step for this lane's DPL disparities (common/sgm_params.h:50-55,
 // sgm_ref.c:61-72 and sgm_ms_ref.c:74-84):
 //   m(d) = min(Lp(d), Lp(d-1)+P1, Lp(d+1)+P1, mn+P2);  L(d) = C(d) + m(d) - mn
@@ -69,24 +108,31 @@ __device__ __forceinline__ uint32_t row_min16(uint32_t v)
 // absent, which the DPP `old` value 0xFFFFFFFF expresses.  SAT clamps the
 // final sum at 255 (multiscale rule; the plain rule cannot exceed 254).
 //
-// edge_lo / edge_hi: Lp+P1 of the disparities just below / above the row's
-// range, seen by lane 0 / lane 15 (0xFFFFFFFF: none).
-template <int DPL, bool SAT>
+// edge_lo / edge_hi: Lp+P1 of the disparities just below / above the line's
+// range, seen by its first / last lane (0xFFFFFFFF: none).  LPL: lanes per
+// scanline (16 or 64, LineLanes).
+template <int DPL, bool SAT, int LPL = kRowLanes>
 __device__ __forceinline__ void sgm_step(uint32_t (&L)[DPL], const uint32_t (&C)[DPL],
                                          uint32_t &mn, uint32_t P1, uint32_t P2,
                                          uint32_t edge_lo = 0xFFFFFFFFu, uint32_t edge_hi = 0xFFFFFFFFu)
 {
+    using LL = LineLanes<LPL>;
     uint32_t a[DPL];
 #pragma unroll
     for (int j = 0; j < DPL; j++)
         a[j] = L[j] + P1;
-    const uint32_t below = dpp_or<kRowShr1>(edge_lo, a[DPL - 1]); // Lp(l*DPL-1)+P1
-    const uint32_t above = dpp_or<kRowShl1>(edge_hi, a[0]);       // Lp(l*DPL+DPL)+P1
+    const uint32_t below = dpp_or<LL::shr>(edge_lo, a[DPL - 1]); // Lp(l*DPL-1)+P1
+    const uint32_t above = dpp_or<LL::shl>(edge_hi, a[0]);       // Lp(l*DPL+DPL)+P1
     const uint32_t jump = mn + P2;
     // -mn through an opaque VALU op: the compiler would fold `C + m + (0 - mn)`
     // back into (C - mn) + m, two adds per disparity instead of one v_add3.
+    // With one line per wave mn is wave-uniform: -mn is a scalar op
+    // (s_mul_i32, which leaves SCC alone) and v_add3 reads it from an SGPR.
     uint32_t neg;
-    asm("v_sub_u32 %0, 0, %1" : "=v"(neg) : "v"(mn));
+    if constexpr (LPL == 64)
+        asm("s_mul_i32 %0, %1, -1" : "=s"(neg) : "s"(__builtin_amdgcn_readfirstlane(mn)));
+    else
+        asm("v_sub_u32 %0, 0, %1" : "=v"(neg) : "v"(mn));
     uint32_t lo = 0xFFFFFFFFu;
 #pragma unroll
     for (int j = 0; j < DPL; j++) {
@@ -99,7 +145,7 @@ __device__ __forceinline__ void sgm_step(uint32_t (&L)[DPL], const uint32_t (&C)
         L[j] = v;
         lo = umin(lo, v);
     }
-    mn = row_min16(lo);
+    mn = LL::min(lo);
 }
 
 // The saturating step on uint16 pairs (D = 32 * NW): lane l's disparities

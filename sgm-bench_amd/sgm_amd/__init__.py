@@ -86,6 +86,7 @@ class Pair(ctypes.Structure):
 
 PLAIN, MULTISCALE = 0, 1
 BATCH_AUTOTUNE, BATCH_NO_GRAPH, BATCH_CENSUS_32X32, BATCH_XCD_SLICES = 1, 2, 4, 8
+LAYOUT_AUTO, LAYOUT_LINES4, LAYOUT_WAVE = 0, 1, 2  # SGMH_LAYOUT_* (include/sgm_hip.h)
 
 
 def hip_lib() -> ctypes.CDLL:
@@ -104,6 +105,7 @@ def hip_lib() -> ctypes.CDLL:
         h.sgmh_ctx_destroy.argtypes = [_P]
         h.sgmh_ctx_destroy.restype = None
         h.sgmh_set_timing.argtypes = [_P, _I]
+        h.sgmh_set_path_layout.argtypes = [_P, _I]
         h.sgmh_get_times.argtypes = [_P, ctypes.POINTER(Times)]
         h.sgmh_plain.argtypes = [_P, _P, _P, _I, _I, _I, _I, _P, ctypes.POINTER(Times)]
         h.sgmh_plain_dev.argtypes = [_P, _P, _P, _I, _I, _I, _I, _P, _P]
@@ -135,7 +137,7 @@ def hip_lib() -> ctypes.CDLL:
         h.sgmh_batch_describe.argtypes = [_P, ctypes.c_char_p, ctypes.c_size_t]
         h.sgmh_batch_destroy.argtypes = [_P]
         h.sgmh_batch_destroy.restype = None
-        for name in ("sgmh_ctx_create", "sgmh_set_timing", "sgmh_get_times", "sgmh_plain",
+        for name in ("sgmh_ctx_create", "sgmh_set_timing", "sgmh_set_path_layout", "sgmh_get_times", "sgmh_plain",
                      "sgmh_plain_dev", "sgmh_plain_noalloc_dev", "sgmh_ms", "sgmh_ms_dev", "sgmh_census_dev",
                      "sgmh_sobel_dev", "sgmh_half_dev", "sgmh_path_plain_dev", "sgmh_copy_probe",
                      "sgmh_batch_plan", "sgmh_batch_run", "sgmh_batch_run_loop",
@@ -338,6 +340,11 @@ class Context:
     # ---- timing ----
     def set_timing(self, on: bool) -> None:
         self._ck(self._lib.sgmh_set_timing(self._h, int(on)))
+
+    def set_path_layout(self, layout: int) -> None:
+        """LAYOUT_AUTO, LAYOUT_LINES4 (four scanlines per wave) or LAYOUT_WAVE
+        (one scanline per wave) for this context's plain path kernel."""
+        self._ck(self._lib.sgmh_set_path_layout(self._h, int(layout)))
 
     def times(self) -> dict:
         t = Times()

@@ -36,9 +36,20 @@ def test_census_descriptors(ctx, H, W):
 DIRS = [(1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (-1, 1), (1, -1), (-1, -1)]
 
 
-@pytest.mark.parametrize("D", [16, 32, 64, 128, 256])
-@pytest.mark.parametrize("H,W", [(23, 77), (40, 300)])
-def test_path_volumes(ctx, D, H, W):
+@pytest.fixture
+def layout_ctx(ctx, request):
+    """The shared context with its plain path layout set for one test."""
+    ctx.set_path_layout(request.param)
+    try:
+        yield ctx
+    finally:
+        ctx.set_path_layout(sgm_amd.LAYOUT_AUTO)
+
+
+LAYOUTS = [pytest.param(sgm_amd.LAYOUT_LINES4, id="lines4"), pytest.param(sgm_amd.LAYOUT_WAVE, id="wave")]
+
+
+def check_path_volumes(ctx, D, H, W):
     L, R = rng_image(H, W, 7 * D + W), rng_image(H, W, 7 * D + W + 1)
     cl, cr = oracle.census(L), oracle.census(R)
     C = oracle.cost(cl, cr, D)
@@ -54,6 +65,33 @@ def test_path_volumes(ctx, D, H, W):
         assert bad.size == 0, f"dir ({dx},{dy}) D={D}: first mismatch at {bad[0]}"
 
 
+@pytest.mark.parametrize("D", [16, 32])
+@pytest.mark.parametrize("H,W", [(23, 77), (40, 300)])
+def test_path_volumes(ctx, D, H, W):
+    check_path_volumes(ctx, D, H, W)
+
+
+# both scanline layouts of the plain path kernel, every direction
+@pytest.mark.parametrize("layout_ctx", LAYOUTS, indirect=True)
+@pytest.mark.parametrize("D", [64, 128, 256])
+@pytest.mark.parametrize("H,W", [(23, 77), (40, 300), (70, 17)])
+def test_path_volumes_layouts(layout_ctx, D, H, W):
+    check_path_volumes(layout_ctx, D, H, W)
+
+
+@pytest.mark.parametrize("D", [16, 32])
+def test_wave_layout_rejects_narrow_D(ctx, D):
+    L, R, _ = sgm_amd.scene_planes(64, 32, D - 1, 5, planes=(3, 6))
+    ctx.set_path_layout(sgm_amd.LAYOUT_WAVE)
+    try:
+        with pytest.raises(sgm_amd.SgmError, match="wave-line layout"):
+            ctx.plain(L, R, D, 4)
+    finally:
+        ctx.set_path_layout(sgm_amd.LAYOUT_AUTO)
+    # the context still works after the error
+    np.testing.assert_array_equal(ctx.plain(L, R, D, 4), oracle.sgm_plain(L, R, D, 4))
+
+
 @pytest.mark.parametrize("H,W,D,paths", [
     (16, 16, 16, 4), (17, 33, 32, 4), (61, 97, 64, 4), (61, 97, 64, 8), (300, 40, 64, 4),
     (120, 200, 128, 4), (120, 200, 128, 8), (20, 1000, 256, 4), (64, 64, 256, 8),
@@ -63,6 +101,17 @@ def test_plain_matches_oracle(ctx, H, W, D, paths):
     L, R, _ = sgm_amd.scene_planes(W, H, min(D - 1, W - 1), W * H + D, planes=(3, 6))
     got = ctx.plain(L, R, D, paths)
     want = oracle.sgm_plain(L, R, D, paths)
+    bad = np.argwhere(got != want)
+    assert bad.size == 0, f"{bad.shape[0]} pixels differ, first at {bad[0]}"
+
+
+@pytest.mark.parametrize("layout_ctx", LAYOUTS, indirect=True)
+@pytest.mark.parametrize("H,W,D", [(61, 97, 64), (300, 40, 64), (120, 200, 128), (20, 1000, 256),
+                                   (515, 77, 64), (480, 640, 64)])
+def test_plain_layouts_match_oracle(layout_ctx, H, W, D):
+    L, R, _ = sgm_amd.scene_planes(W, H, min(D - 1, W - 1), W * H + 3 * D, planes=(3, 6))
+    got = layout_ctx.plain(L, R, D, 4)
+    want = oracle.sgm_plain(L, R, D, 4)
     bad = np.argwhere(got != want)
     assert bad.size == 0, f"{bad.shape[0]} pixels differ, first at {bad[0]}"
 
@@ -95,6 +144,20 @@ def test_computed_goldens_plain(ctx, goldens):
         L, R = make_scene(g["scene"])
         assert sgm_amd.fnv1a64(np.concatenate([L.ravel(), R.ravel()])) == g["input_fnv"], g["name"]
         out = ctx.plain(L, R, g["D"], g["paths"])
+        assert sgm_amd.fnv1a64(out) == g["fnv"], g["name"]
+
+
+@pytest.mark.parametrize("layout_ctx", LAYOUTS, indirect=True)
+def test_goldens_both_layouts(layout_ctx, goldens):
+    """Every plain golden with D = 64, 128 or 256 (the reference's on-disk ones
+    and the compiled reference's hashes, 4 and 8 paths, up to 1920x1080) with
+    the path kernel's scanline layout forced."""
+    gs = [g for g in goldens["reference_goldens"] + goldens["computed"] + goldens["ties"]
+          if g["mode"] == "plain" and g["D"] % 64 == 0]
+    assert len(gs) >= 20
+    for g in gs:
+        L, R = make_scene(g["scene"])
+        out = layout_ctx.plain(L, R, g["D"], g["paths"])
         assert sgm_amd.fnv1a64(out) == g["fnv"], g["name"]
 
 
