@@ -111,20 +111,46 @@ constexpr long kWaveLinesMaxPixelsD128 = 1280L * 720; // D = 128 frames up to th
 
 // LPL: lanes per scanline, 16 (four lines per wave) or 64 (the wave-line
 // layout, one line per wave, plain census-fed paths only).
+// Block `block` of a launch over the direction table of a: its direction, its
+// line group, then the group's scanlines.
 template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, bool BCAST = false, int LPL = kRowLanes>
-__global__ __launch_bounds__(64) void k_path(PathArgs a)
+__device__ __forceinline__ void path_block(const PathArgs &a, int block)
 {
     constexpr int LPW = 64 / LPL;
     int k = 0;
-    while (k + 1 < a.ndirs && (int)blockIdx.x >= a.dir[k + 1].first_block)
+    while (k + 1 < a.ndirs && block >= a.dir[k + 1].first_block)
         k++;
     const DirTask t = a.dir[k];
-    const int i = (int)blockIdx.x - t.first_block;
+    const int i = block - t.first_block;
     const int n8 = (t.nlines + LPW * kXcds - 1) / (LPW * kXcds) * kXcds;
     const int grp = t.dx == 0 ? (i % kXcds) * (n8 / kXcds) + i / kXcds : i;
     if (grp * LPW >= t.nlines)
         return; // padding block
     path_group<DPL, FROM_CENSUS, SAT, DECIM, PathArgs, BCAST, LPL>(a, t, grp);
+}
+
+template <int DPL, bool FROM_CENSUS, bool SAT, bool DECIM, bool BCAST = false, int LPL = kRowLanes>
+__global__ __launch_bounds__(64) void k_path(PathArgs a)
+{
+    path_block<DPL, FROM_CENSUS, SAT, DECIM, BCAST, LPL>(a, (int)blockIdx.x);
+}
+
+// The two multiscale stages' 8-path aggregations in one launch: they read
+// different cost volumes (C2 already holds the merge with C1) and write
+// different volumes, so nothing orders them.  Blocks [0, nhalf) run the
+// half-scale stage, the rest the full-scale stage (decimated store).  The
+// half-scale stage alone leaves the GPU short of waves and is bound by its
+// lines' step chains; dispatched first, those chains run beside the
+// full-scale work (measured at 1920x1080: aggregation 0.507 ms, against 0.568
+// with the full-scale blocks first and 0.597 as two launches).  nhalf is a
+// multiple of the XCD count, so each stage's blocks keep their XCD dealing.
+template <int DPL>
+__global__ __launch_bounds__(64) void k_path_ms(PathArgs full, PathArgs half, int nhalf)
+{
+    if ((int)blockIdx.x < nhalf)
+        path_block<DPL, false, true, false>(half, (int)blockIdx.x);
+    else
+        path_block<DPL, false, true, true>(full, (int)blockIdx.x - nhalf);
 }
 
 template <int LPP, int NDIRS>
@@ -228,9 +254,10 @@ static int launch_path_wave(hipStream_t s, const PathArgs &a, int blocks)
 // so the first blocks the dispatcher places are the longest-running ones.
 // Two variants are instantiated: census-fed plain paths and cost-volume-fed
 // saturating multiscale paths (full or decimated store).
-static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndirs,
-                        bool from_census, bool sat, uint8_t *vol, size_t dir_stride, int decimate,
-                        bool wave_lines = false)
+// fill_paths builds a launch's direction table in a and returns its block
+// count (-1 on a size error); launch_paths launches it.
+static int fill_paths(PathArgs &a, const int (*dirs)[3], int ndirs, bool from_census, bool sat,
+                      uint8_t *vol, size_t dir_stride, int decimate, bool wave_lines)
 {
     REQUIRE(!wave_lines || (from_census && !sat && !decimate), "sgmh: the wave-line layout is for plain paths");
     const int lpw = wave_lines ? 1 : kLinesPerWave; // scanlines per wave
@@ -271,6 +298,16 @@ static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndi
         const int groups = (t.nlines + lpw - 1) / lpw;
         blocks += (groups + kXcds - 1) / kXcds * kXcds;
     }
+    return blocks;
+}
+
+static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndirs,
+                        bool from_census, bool sat, uint8_t *vol, size_t dir_stride, int decimate,
+                        bool wave_lines = false)
+{
+    const int blocks = fill_paths(a, dirs, ndirs, from_census, sat, vol, dir_stride, decimate, wave_lines);
+    if (blocks < 0)
+        return -1;
     if (wave_lines)
         return launch_path_wave(s, a, blocks);
     // the broadcast row variant pays where row waves are at least one per CU,
@@ -284,6 +321,28 @@ static int launch_paths(hipStream_t s, PathArgs a, const int (*dirs)[3], int ndi
         return launch_path_t<false, true, true>(s, a, blocks);
     set_err("sgmh: unsupported path-kernel variant");
     return -1;
+}
+
+// Both multiscale stages' paths (k_path_ms): `full` over C2 with the
+// decimated store into V2, `half` over C1 into V1.
+static int launch_paths_ms(hipStream_t s, PathArgs full, PathArgs half, const int (*dirs)[3], uint8_t *V2,
+                           uint8_t *V1, size_t dir_stride)
+{
+    const int nfull = fill_paths(full, dirs, 8, false, true, V2, dir_stride, 1, false);
+    const int nhalf = nfull < 0 ? -1 : fill_paths(half, dirs, 8, false, true, V1, dir_stride, 0, false);
+    if (nhalf < 0)
+        return -1;
+    const dim3 grid(nfull + nhalf);
+    switch (full.D / 16) {
+    case 1: hipLaunchKernelGGL(k_path_ms<1>, grid, dim3(64), 0, s, full, half, nhalf); break;
+    case 2: hipLaunchKernelGGL(k_path_ms<2>, grid, dim3(64), 0, s, full, half, nhalf); break;
+    case 4: hipLaunchKernelGGL(k_path_ms<4>, grid, dim3(64), 0, s, full, half, nhalf); break;
+    case 8: hipLaunchKernelGGL(k_path_ms<8>, grid, dim3(64), 0, s, full, half, nhalf); break;
+    case 16: hipLaunchKernelGGL(k_path_ms<16>, grid, dim3(64), 0, s, full, half, nhalf); break;
+    default: set_err("sgmh: bad D %d", full.D); return -1;
+    }
+    CK(hipGetLastError());
+    return 0;
 }
 
 template <int LPP>
@@ -619,15 +678,14 @@ static int ms_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint
     a.invalid = kMsInvalid;
     {
         PhaseScope ps(ctx, s, PH_AGG);
-        a.cost = C1;
-        a.W = hw;
-        a.H = hh;
-        if (launch_paths(s, a, dirs, 8, false, true, V1, Nh * D, 0))
-            return -1;
+        PathArgs half = a;
+        half.cost = C1;
+        half.W = hw;
+        half.H = hh;
         a.cost = C2;
         a.W = W;
         a.H = H;
-        if (launch_paths(s, a, dirs, 8, false, true, V2, Nh * D, 1))
+        if (launch_paths_ms(s, a, half, dirs, V2, V1, Nh * D))
             return -1;
     }
     {
