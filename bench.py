@@ -108,15 +108,17 @@ def golden_for(seed, mode, D, paths, W, H):
 def kernel_bytes(mode, W, H, D, paths):
     """Algorithmic HBM bytes of one launch of the path-aggregation kernel: each
     (pixel, direction) writes its D path costs once as uint8 and reads the two
-    64-bit census codes of that pixel once (plain); multiscale reads the uint8
-    cost volume once per direction and writes D bytes per direction only at the
-    pixels the decimated argmin reads, at both scales."""
+    64-bit census codes of that pixel once (plain).  Multiscale reads the uint8
+    cost volume once per direction and pixel of the lines it aggregates and
+    writes D bytes per direction at the pixels the argmin reads: every line at
+    half scale; at full scale only the lines through even (x, y) -- half of
+    each direction's, the others store nothing the decimated argmin reads."""
     N = W * H
     if mode == "plain":
         return paths * (N * D + 16 * N)
     Nh = N // 4
-    stage1 = 8 * (Nh * D + Nh * D)      # read C1, write all of L1
-    stage2 = 8 * (N * D + Nh * D)       # read C2, write L2 at even pixels
+    stage1 = 8 * (Nh * D + Nh * D)        # read C1, write all of L1
+    stage2 = 8 * (N // 2 * D + Nh * D)    # read C2 along half the lines, write L2 at even pixels
     return stage1 + stage2
 
 

@@ -370,20 +370,11 @@ long group_work(const BatchPathTask &t, int grp)
         return (long)H * t.D;
     long best = 0;
     for (int q = 0; q < kLinesPerWave; q++) {
-        const int i = grp * kLinesPerWave + q;
-        if (i >= t.dir.nlines)
+        const int m = grp * kLinesPerWave + q;
+        if (m >= t.dir.nlines)
             break;
-        int x, y;
-        if (i < W) {
-            x = dx > 0 ? i : W - 1 - i;
-            y = dy > 0 ? 0 : H - 1;
-        } else {
-            const int j = i - W + 1;
-            x = dx > 0 ? 0 : W - 1;
-            y = dy > 0 ? j : H - 1 - j;
-        }
-        const int sx = dx > 0 ? W - x : x + 1, sy = dy > 0 ? H - y : y + 1;
-        best = std::max<long>(best, std::min(sx, sy));
+        const int i = t.dir.dec ? dec_line(m, dx, dy, W) : m;
+        best = std::max<long>(best, line_geom(i, dx, dy, W, H).steps);
     }
     return best * t.D;
 }
@@ -693,7 +684,9 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
             t.dir.dx = dx;
             t.dir.dy = dy;
             t.dir.P1 = P1;
-            t.dir.nlines = dy == 0 ? H : (dx == 0 ? W : W + H - 1);
+            // the decimated stage aggregates only the lines through even (x, y)
+            t.dir.dec = variant >= kVarMsDec32 ? 1 : 0;
+            t.dir.nlines = t.dir.dec ? dec_lines(dx, dy, W, H) : (dy == 0 ? H : (dx == 0 ? W : W + H - 1));
             t.dir.first_block = 0;
             t.dir.L = L;
             t.variant = variant;

@@ -414,8 +414,9 @@ __device__ __forceinline__ void ld_cost(rsrc_t r, uint32_t off, CostWords<DPL> &
 
 struct DirTask {
     int dx, dy, P1;
-    int nlines;      // scanlines in this direction
+    int nlines;      // scanlines in this direction (dec: the surviving ones, dec_lines)
     int first_block; // first block of this direction in the grid
+    int dec;         // 1: the decimated multiscale stage, which aggregates only dec_line's lines
     uint8_t *L;      // output volume of this direction
 };
 
@@ -443,7 +444,7 @@ struct LineGeom {
 // axis-aligned direction are the rows / columns; a diagonal direction has
 // W+H-1 lines starting on the edge row its predecessors fall off, then on the
 // edge column (multiscale/sgm_ms_cuda.cu:91-98 enumerates the same set).
-__device__ __forceinline__ LineGeom line_geom(int i, int dx, int dy, int W, int H)
+__host__ __device__ __forceinline__ LineGeom line_geom(int i, int dx, int dy, int W, int H)
 {
     if (dy == 0)
         return {dx > 0 ? 0 : W - 1, i, W};
@@ -460,6 +461,34 @@ __device__ __forceinline__ LineGeom line_geom(int i, int dx, int dy, int W, int 
     }
     const int sx = dx > 0 ? W - x : x + 1, sy = dy > 0 ? H - y : y + 1;
     return {x, y, sx < sy ? sx : sy};
+}
+
+// The decimated multiscale stage keeps L only at even (x, y)
+// (multiscale/sgm_ms_ref.c:194-199), and a scanline's L depends on the earlier
+// pixels of that line alone.  So only the lines through even-even pixels are
+// aggregated: the even rows, the even columns, and the diagonals whose x + y
+// is even (a diagonal step changes x + y by 0 or 2).  The other half of every
+// direction's lines store nothing and feed nothing.  W and H are even there.
+// dec_lines: how many lines survive; dec_line: the line_geom index of
+// surviving line m.  A diagonal's first W lines start on the edge row at x = i
+// or W-1-i, the next H-1 on the edge column at row j = i-W+1 or H-1-j; with
+// a = (dx < 0) + (dy < 0) mod 2, x + y is even exactly for i = a mod 2 on the
+// edge row and j = a mod 2 on the edge column.
+__host__ __device__ __forceinline__ int dec_lines(int dx, int dy, int W, int H)
+{
+    if (dy == 0)
+        return H / 2;
+    if (dx == 0)
+        return W / 2;
+    const int a = ((dx < 0) + (dy < 0)) & 1;
+    return W / 2 + H / 2 - 1 + a;
+}
+__host__ __device__ __forceinline__ int dec_line(int m, int dx, int dy, int W)
+{
+    if (dy == 0 || dx == 0)
+        return 2 * m;
+    const int a = ((dx < 0) + (dy < 0)) & 1;
+    return m < W / 2 ? 2 * m + a : W - 1 + 2 * (m - W / 2) + 2 - a;
 }
 
 // 64-bit row shift by one lane; the lane without a source (lane 0 for shr,
@@ -788,15 +817,17 @@ __device__ __forceinline__ void path_group(const A &a, const DirTask &t, int grp
     constexpr int LPW = 64 / LPL; // scanlines per wave
     const int lane = threadIdx.x & 63, g = lane / LPL, l = lane % LPL;
     const int base = grp * LPW;
+    // line_geom's index of this direction's line m
+    auto index = [&](int m) { return t.dec ? dec_line(m, t.dx, t.dy, a.W) : m; };
     int maxsteps = 0;
 #pragma unroll
     for (int q = 0; q < LPW; q++)
         if (base + q < t.nlines)
-            maxsteps = max(maxsteps, line_geom(base + q, t.dx, t.dy, a.W, a.H).steps);
+            maxsteps = max(maxsteps, line_geom(index(base + q), t.dx, t.dy, a.W, a.H).steps);
     const int line = base + g;
     LineGeom ln = {0, 0, 0};
     if (line < t.nlines)
-        ln = line_geom(line, t.dx, t.dy, a.W, a.H);
+        ln = line_geom(index(line), t.dx, t.dy, a.W, a.H);
     if constexpr (FROM_CENSUS && !DECIM) {
         if (t.dy == 0) {
             if (t.dx > 0)

@@ -291,7 +291,9 @@ static int fill_paths(PathArgs &a, const int (*dirs)[3], int ndirs, bool from_ce
         t.dx = dirs[k][0];
         t.dy = dirs[k][1];
         t.P1 = dirs[k][2];
-        t.nlines = t.dy == 0 ? a.H : (t.dx == 0 ? a.W : a.W + a.H - 1);
+        t.dec = decimate ? 1 : 0;
+        t.nlines = decimate ? dec_lines(t.dx, t.dy, a.W, a.H)
+                            : (t.dy == 0 ? a.H : (t.dx == 0 ? a.W : a.W + a.H - 1));
         t.first_block = blocks;
         t.L = vol + (size_t)k * dir_stride;
         // line groups of this direction, padded to a multiple of the XCD count
@@ -601,6 +603,7 @@ int window_rest(sgmh_ctx *ctx, hipStream_t s, int W, int H, int D, const uint8_t
         t.dy = kDirsPlain[o][1];
         t.P1 = (int)kPlainP1;
         t.nlines = t.dy == 0 ? H : W;
+        t.dec = 0;
         t.first_block = blocks;
         t.L = vol + (size_t)k * N * D;
         a.Ln[k] = nvol + (size_t)k * N * 32;
