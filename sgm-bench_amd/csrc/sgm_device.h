@@ -85,6 +85,10 @@ __device__ __forceinline__ uint32_t and_or(uint32_t a, uint32_t m, uint32_t b)
 // is taken into bit 8 + n % 8 of word n / 8 by one v_and_or_b32, so word k
 // ends up holding code byte k of the pair's first pixel in its byte 1 and of
 // the second pixel in its byte 3; byte permutes then assemble the codes.
+template <int TX, int TY>
+__device__ __forceinline__ void census_from_tile(uint64_t *__restrict__ out, int W, int H, int ostride, int x0,
+                                                 int y0, const uint32_t *__restrict__ tile);
+
 template <int TX = 64, int TY = 16>
 __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uint64_t *__restrict__ out,
                                             int W, int H, int ostride, int x0, int y0,
@@ -97,7 +101,7 @@ __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uin
     // of 4 (and an image base aligned to 4 bytes) is staged by aligned 4-byte
     // words; the others by clamped bytes.
     using T = CenTile<TX, TY>;
-    constexpr int kCenRows = T::Rows, kCenRowBytes = T::RowBytes, kCenRowWords = T::RowWords;
+    constexpr int kCenRows = T::Rows, kCenRowBytes = T::RowBytes;
     if (x0 >= 4 && x0 + TX + 4 <= W && (W & 3) == 0 && (reinterpret_cast<uintptr_t>(img) & 3) == 0) {
         constexpr int kWords = kCenRows * (kCenRowBytes / 4), kIters = (kWords + 255) / 256;
         uint32_t wv[kIters];
@@ -140,6 +144,18 @@ __device__ __forceinline__ void census_tile(const uint8_t *__restrict__ img, uin
         }
     }
     __syncthreads();
+    census_from_tile<TX, TY>(out, W, H, ostride, x0, y0, tile);
+}
+
+// The codes of a TX x TY output tile from its staged window (CenTile layout,
+// one uint16 per pixel), 4 consecutive pixels per thread.  Every thread of the
+// block calls it after the staging barrier.
+template <int TX, int TY>
+__device__ __forceinline__ void census_from_tile(uint64_t *__restrict__ out, int W, int H, int ostride, int x0,
+                                                 int y0, const uint32_t *__restrict__ tile)
+{
+    using T = CenTile<TX, TY>;
+    constexpr int kCenRowWords = T::RowWords;
     const int tx = threadIdx.x & (TX / 4 - 1), ty = threadIdx.x / (TX / 4);
     const int x = x0 + 4 * tx, y = y0 + ty;
     if (y >= H || x >= W)
@@ -215,6 +231,65 @@ __device__ __forceinline__ void half_px(const uint8_t *__restrict__ img, int W, 
 {
     const uint8_t *a = img + (size_t)(2 * y) * W + 2 * x;
     out[(size_t)y * w + x] = (uint8_t)((a[0] + a[1] + a[W] + a[W + 1] + 2) >> 2);
+}
+
+// The multiscale census input staged straight into a 64x16 census tile:
+// Sobel-X (multiscale/sgm_ms_ref.c:47-54) of the image, or of its 2x2 box
+// mean (HALF, sgm_ms_ref.c:145-150), at the census window's clamped
+// coordinates, so neither the half-scale image nor the Sobel image is
+// written.  W, H: the input image; the codes are of the (W/2)x(H/2) half-scale
+// image when HALF.  win: LDS for the scaled image over the window's Sobel
+// neighbourhood, (64+10) x (16+8) bytes.  Then the codes as census_tile makes
+// them.  Every thread of the block calls it.
+constexpr int kSobWinX = kCenTX + 10, kSobWinY = kCenTY + 8;
+template <bool HALF>
+__device__ __forceinline__ void sobel_census_tile(const uint8_t *__restrict__ img, int W, int H,
+                                                  uint64_t *__restrict__ out, int x0, int y0,
+                                                  uint32_t *__restrict__ tile, uint8_t *__restrict__ win)
+{
+    const int w = HALF ? W / 2 : W, h = HALF ? H / 2 : H;
+    constexpr int kWin = kSobWinX * kSobWinY, kIters = (kWin + 255) / 256;
+    // the scaled image over [x0-5, x0+69] x [y0-4, y0+19], clamped: every load
+    // is issued before the first LDS write
+    uint32_t v[kIters];
+#pragma unroll
+    for (int k = 0; k < kIters; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        const int r = i / kSobWinX, c = i - r * kSobWinX;
+        const int X = clampi(x0 - 5 + c, 0, w - 1), Y = clampi(y0 - 4 + r, 0, h - 1);
+        if (k + 1 < kIters || i < kWin) {
+            if constexpr (HALF) {
+                const uint8_t *a = img + (size_t)(2 * Y) * W + 2 * X;
+                v[k] = ((uint32_t)a[0] + a[1] + a[W] + a[W + 1] + 2) >> 2;
+            } else {
+                v[k] = img[(size_t)Y * W + X];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kIters; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        if (k + 1 < kIters || i < kWin)
+            win[i] = (uint8_t)v[k];
+    }
+    __syncthreads();
+    // Sobel-X at each census window pixel's clamped (cx, cy), from the scaled
+    // pixels at clamped cx +- 1, cy +- 1 (all inside win)
+    uint16_t *t16 = reinterpret_cast<uint16_t *>(tile);
+    constexpr int kTile = kCenRows * kCenRowBytes;
+    for (int i = (int)threadIdx.x; i < kTile; i += 256) {
+        const int r = i / kCenRowBytes, c = i - r * kCenRowBytes;
+        const int cx = clampi(x0 - 4 + c, 0, w - 1), cy = clampi(y0 - 3 + r, 0, h - 1);
+        const int xl = clampi(cx - 1, 0, w - 1) - (x0 - 5), xr = clampi(cx + 1, 0, w - 1) - (x0 - 5);
+        const int yu = (clampi(cy - 1, 0, h - 1) - (y0 - 4)) * kSobWinX;
+        const int ym = (cy - (y0 - 4)) * kSobWinX;
+        const int yd = (clampi(cy + 1, 0, h - 1) - (y0 - 4)) * kSobWinX;
+        const int gx = ((int)win[yu + xr] - win[yu + xl]) + 2 * ((int)win[ym + xr] - win[ym + xl]) +
+                       ((int)win[yd + xr] - win[yd + xl]);
+        t16[i] = (uint16_t)clampi(128 + (gx >> 3), 0, 255);
+    }
+    __syncthreads();
+    census_from_tile<kCenTX, kCenTY>(out, w, h, w, x0, y0, tile);
 }
 
 // Sobel-X, out = clamp(128 + (gx >> 3)) with arithmetic shift,
@@ -726,10 +801,9 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
     for (int j = 0; j < DPL; j++)
         L[j] = 0;
 
-    // One block of PF steps; CHECK: a step of the block may lie past the end
-    // of its line (always possible on diagonals, whose lines differ in length).
-    // census-fed costs of ring slot k, at x = xk
-    auto costs = [&](int k, int xk, uint32_t (&C)[DPL]) {
+    // census-fed costs of ring slot k, at x = xk (a generic lambda: formed only
+    // where it is called, in the census-fed instantiations)
+    auto costs = [&](int k, int xk, auto &C) {
         const uint64_t cL = rq[k][DPL];
         const int rel = xk - lbase;
 #pragma unroll
@@ -743,6 +817,8 @@ __device__ __forceinline__ void path_generic(const A &a, const DirTask &t, LineG
                 C[j] = rel >= j ? h : inval;
         }
     };
+    // One block of PF steps; CHECK: a step of the block may lie past the end
+    // of its line (always possible on diagonals, whose lines differ in length).
     auto block = [&](int s0, auto check_tag) {
         constexpr bool CHECK = decltype(check_tag)::value;
 #pragma unroll

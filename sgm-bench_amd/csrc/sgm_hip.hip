@@ -8,9 +8,12 @@
 //                   written as uint8 to its own volume
 //   k_wta<LPP,..>   S = sum of the direction volumes (uint16 semantics) and
 //                   argmin with the lowest d winning ties, fused
-// Multiscale (multiscale/sgm_ms_ref.c:166-202) adds k_half, k_sobel and
-// k_cost (materialised cost volume, merged across scales) and runs k_path in
-// its cost-volume mode with saturation and direction-weighted P1.
+// Multiscale (multiscale/sgm_ms_ref.c:166-202) makes the census of the
+// Sobel images at both scales in one launch (k_census_ms, half-scale and Sobel
+// filters fused into the census tile), adds k_cost (materialised cost volume,
+// merged across scales) and runs both stages' paths in one launch
+// (k_path_ms) in the cost-volume mode with saturation and direction-weighted
+// P1.  k_half and k_sobel remain as the filters' own entry points.
 // The device code itself is in sgm_device.h; the persistent batch kernels
 // that reuse it are in sgm_batch.hip.
 #include <hip/hip_runtime.h>
@@ -49,6 +52,29 @@ __global__ __launch_bounds__(256) void k_census(const uint8_t *__restrict__ img0
     const bool second = blockIdx.z != 0;
     census_tile(second ? img1 : img0, second ? out1 : out0, W, H, ostride, blockIdx.x * kCenTX,
                 blockIdx.y * kCenTY, tile);
+}
+
+// The multiscale census of both images at both scales in one launch
+// (sobel_census_tile): z = 0, 1 the half scale of left, right; z = 2, 3 the
+// full scale.  The grid covers the full scale; the half scale's extra blocks
+// return at once.
+__global__ __launch_bounds__(256) void k_census_ms(const uint8_t *__restrict__ l, const uint8_t *__restrict__ r,
+                                                   int W, int H, uint64_t *__restrict__ cl1,
+                                                   uint64_t *__restrict__ cr1, uint64_t *__restrict__ cl2,
+                                                   uint64_t *__restrict__ cr2)
+{
+    __shared__ __align__(16) uint32_t tile[kCenRows * kCenRowWords];
+    __shared__ uint8_t win[kSobWinX * kSobWinY];
+    const int z = (int)blockIdx.z;
+    const bool half = z < 2, right = (z & 1) != 0;
+    const int x0 = (int)blockIdx.x * kCenTX, y0 = (int)blockIdx.y * kCenTY;
+    if (x0 >= (half ? W / 2 : W) || y0 >= (half ? H / 2 : H))
+        return;
+    const uint8_t *img = right ? r : l;
+    if (half)
+        sobel_census_tile<true>(img, W, H, right ? cr1 : cl1, x0, y0, tile, win);
+    else
+        sobel_census_tile<false>(img, W, H, right ? cr2 : cl2, x0, y0, tile, win);
 }
 
 __global__ __launch_bounds__(256) void k_half(const uint8_t *__restrict__ img, int W, int H,
@@ -638,14 +664,11 @@ static int ms_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint
 {
     const int hw = W / 2, hh = H / 2;
     const size_t N = (size_t)W * H, Nh = (size_t)hw * hh;
-    if (ensure(ctx->half_l, Nh) || ensure(ctx->half_r, Nh) || ensure(ctx->sob_l, N) ||
-        ensure(ctx->sob_r, N) || ensure(ctx->cl1, Nh * 8) || ensure(ctx->cr1, Nh * 8) ||
+    if (ensure(ctx->cl1, Nh * 8) || ensure(ctx->cr1, Nh * 8) ||
         ensure(ctx->cl, N * 8) || ensure(ctx->cr, N * 8) || ensure(ctx->cost1, Nh * D) ||
         ensure(ctx->cost2, N * D) || ensure(ctx->vol1, Nh * D * 8) || ensure(ctx->vol, Nh * D * 8) ||
         (disp1 == nullptr && ensure(ctx->disp1, Nh)))
         return -1;
-    uint8_t *hl = (uint8_t *)ctx->half_l.p, *hr = (uint8_t *)ctx->half_r.p;
-    uint8_t *sl = (uint8_t *)ctx->sob_l.p, *sr = (uint8_t *)ctx->sob_r.p;
     uint64_t *cl1 = (uint64_t *)ctx->cl1.p, *cr1 = (uint64_t *)ctx->cr1.p;
     uint64_t *cl2 = (uint64_t *)ctx->cl.p, *cr2 = (uint64_t *)ctx->cr.p;
     uint8_t *C1 = (uint8_t *)ctx->cost1.p, *C2 = (uint8_t *)ctx->cost2.p;
@@ -655,19 +678,9 @@ static int ms_enqueue(sgmh_ctx *ctx, hipStream_t s, const uint8_t *l, const uint
     memcpy(dirs, kDirsMs, sizeof dirs);
     {
         PhaseScope ps(ctx, s, PH_CENSUS);
-        const dim3 gh((hw + 63) / 64, (hh + 3) / 4), gf((W + 63) / 64, (H + 3) / 4);
-        hipLaunchKernelGGL(k_half, gh, dim3(256), 0, s, l, W, H, hl);
-        hipLaunchKernelGGL(k_half, gh, dim3(256), 0, s, r, W, H, hr);
-        hipLaunchKernelGGL(k_sobel, dim3((hw + 63) / 64, (hh + 3) / 4), dim3(256), 0, s, hl, hw, hh, sl);
-        hipLaunchKernelGGL(k_sobel, dim3((hw + 63) / 64, (hh + 3) / 4), dim3(256), 0, s, hr, hw, hh, sr);
+        const dim3 grid((W + kCenTX - 1) / kCenTX, (H + kCenTY - 1) / kCenTY, 4);
+        hipLaunchKernelGGL(k_census_ms, grid, dim3(256), 0, s, l, r, W, H, cl1, cr1, cl2, cr2);
         CK(hipGetLastError());
-        if (launch_census(s, sl, cl1, hw, hh, hw, sr, cr1))
-            return -1;
-        hipLaunchKernelGGL(k_sobel, gf, dim3(256), 0, s, l, W, H, sl);
-        hipLaunchKernelGGL(k_sobel, gf, dim3(256), 0, s, r, W, H, sr);
-        CK(hipGetLastError());
-        if (launch_census(s, sl, cl2, W, H, W, sr, cr2))
-            return -1;
     }
     {
         PhaseScope ps(ctx, s, PH_COST);
@@ -737,8 +750,7 @@ void sgmh_ctx_destroy(sgmh_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     DevBuf *bufs[] = {&c->img_l, &c->img_r, &c->disp, &c->disp1, &c->cl, &c->cr, &c->vol,
-                      &c->half_l, &c->half_r, &c->sob_l, &c->sob_r, &c->cl1, &c->cr1,
-                      &c->cost1, &c->cost2, &c->vol1};
+                      &c->cl1, &c->cr1, &c->cost1, &c->cost2, &c->vol1};
     for (DevBuf *b : bufs)
         free_buf(*b);
     for (hipEvent_t e : c->events)

@@ -51,7 +51,6 @@ struct sgmh_ctx {
     DevBuf img_l, img_r, disp, disp1;          // host-buffer entry points
     DevBuf cl, cr;                             // census codes
     DevBuf vol;                                // direction volumes
-    DevBuf half_l, half_r, sob_l, sob_r;       // multiscale images
     DevBuf cl1, cr1, cost1, cost2, vol1;       // multiscale stage buffers
 };
 
