@@ -1101,6 +1101,11 @@ struct WinState {
         for (int j = 0; j < n; j++)
             v[j] = b[j];
     }
+    // The edges are formed as values and assigned on every path: a store to
+    // one member on one branch and to another on the other lets the optimiser
+    // merge them into one store through a pointer select, which puts the
+    // whole state in scratch memory (and a scratch load waits for every
+    // prefetch in flight).
     __device__ __forceinline__ void exchange(uint8_t *row, int l, uint32_t P1, const WinSeg &o, const WinSeg &c)
     {
         if (o.mode == 0) {
@@ -1113,14 +1118,17 @@ struct WinState {
         } else {
             put(row + NB * l, SF, NB);
         }
+        uint32_t lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
         if (c.mode == 0) {
             get(row + 16 * c.b0 + 2 * l, SN, 2);
             const int dl = 16 * c.b0 - 1, dh = 16 * c.b0 + 32;
-            edge_lo = dl >= 0 ? (uint32_t)row[dl] + P1 : 0xFFFFFFFFu;
-            edge_hi = dh < D ? (uint32_t)row[dh] + P1 : 0xFFFFFFFFu;
+            lo = dl >= 0 ? (uint32_t)row[dl] + P1 : 0xFFFFFFFFu;
+            hi = dh < D ? (uint32_t)row[dh] + P1 : 0xFFFFFFFFu;
         } else {
             get(row + NB * l, SF, NB);
         }
+        edge_lo = lo;
+        edge_hi = hi;
     }
 };
 
@@ -1280,10 +1288,11 @@ __device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask
                                    i == 0 ? S.edge_hi : 0xFFFFFFFFu);
                 const uint32_t off = real ? sb_n + (uint32_t)(32 * sstep(i)) : kDrop;
                 __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(S.SN[0] | (S.SN[1] << 8)), rn, off, 0, 0);
-                if (CHECK && q < q0) { // before the line's first pixel (backward lines): keep the start state
-                    S.SN[0] = 0;
-                    S.SN[1] = 0;
-                    S.mn = 0;
+                if constexpr (CHECK) { // before the line's first pixel (backward lines): keep the start state
+                    const bool pre = q < q0;
+                    S.SN[0] = pre ? 0u : S.SN[0];
+                    S.SN[1] = pre ? 0u : S.SN[1];
+                    S.mn = pre ? 0u : S.mn;
                 }
             } else {
                 sgm_step<NB, false>(S.SF, C, S.mn, P1, P2);
@@ -1296,11 +1305,12 @@ __device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask
                 }
                 const uint32_t off = real ? sb_f + (uint32_t)(D * sstep(i)) : kDrop;
                 st_bytes<NB, (NB >= 8)>(ro, off, S.SF);
-                if (CHECK && q < q0) {
+                if constexpr (CHECK) {
+                    const bool pre = q < q0;
 #pragma unroll
                     for (int j = 0; j < NB; j++)
-                        S.SF[j] = 0;
-                    S.mn = 0;
+                        S.SF[j] = pre ? 0u : S.SF[j];
+                    S.mn = pre ? 0u : S.mn;
                 }
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -1309,29 +1319,55 @@ __device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask
         S.edge_hi = 0xFFFFFFFFu;
     };
 
-#pragma nounroll
-    for (int k = 0; k < nt; k++) {
+    // boundary segments: steps outside the line, x within D of the row start
+    auto variant = [&](int k) {
+        const int xmin = DIR > 0 ? 16 * k : 16 * nt - 16 - 16 * k;
+        const bool check = 16 * k < q0 || 16 * k + 16 > qend || xmin < D || cur.mode == 2;
+        return (cur.mode == 0 ? 0 : 2) + (check ? 1 : 0);
+    };
+    // One segment of a run of the variant V (mode and boundary tests), then
+    // the move to the next segment.
+    auto one = [&](int k, auto v_tag) {
+        constexpr int V = decltype(v_tag)::value;
         if (k > 0 && win_changes<NB>(prev, cur))
             S.exchange(xs + g * D, l, P1, prev, cur);
         bases(k, cur, nxt);
-        // boundary segments: steps outside the line, x within D of the row start
-        const int xmin = DIR > 0 ? 16 * k : 16 * nt - 16 - 16 * k;
-        const bool check = 16 * k < q0 || 16 * k + 16 > qend || xmin < D || cur.mode == 2;
-        if (cur.mode == 0) {
-            if (check)
-                segment(k, std::integral_constant<int, 0>{}, std::true_type{});
-            else
-                segment(k, std::integral_constant<int, 0>{}, std::false_type{});
-        } else {
-            if (check)
-                segment(k, std::integral_constant<int, 1>{}, std::true_type{});
-            else
-                segment(k, std::integral_constant<int, 1>{}, std::false_type{});
-        }
+        segment(k, std::integral_constant<int, (V >> 1)>{}, std::integral_constant<bool, (V & 1) != 0>{});
         prev = cur;
         cur = nxt;
         nxt = win_seg<NB>(__builtin_amdgcn_readfirstlane(raw_nn));
         raw_nn = seg_raw(k + 3);
+    };
+    // Segments run in maximal runs of one variant, each run its own loop.  A
+    // variant's unrolled body leaves the prefetch ring in registers of its own
+    // choosing; where two variants' bodies meet, the copies between them wait
+    // for every load in flight.  With one loop per run that happens only
+    // where the variant changes, not at every segment.
+    int k = 0;
+#pragma nounroll
+    while (k < nt) {
+        const int v = variant(k);
+        if (v == 0) {
+#pragma nounroll
+            do
+                one(k++, std::integral_constant<int, 0>{});
+            while (k < nt && variant(k) == 0);
+        } else if (v == 1) {
+#pragma nounroll
+            do
+                one(k++, std::integral_constant<int, 1>{});
+            while (k < nt && variant(k) == 1);
+        } else if (v == 2) {
+#pragma nounroll
+            do
+                one(k++, std::integral_constant<int, 2>{});
+            while (k < nt && variant(k) == 2);
+        } else {
+#pragma nounroll
+            do
+                one(k++, std::integral_constant<int, 3>{});
+            while (k < nt && variant(k) == 3);
+        }
     }
 }
 
@@ -1452,10 +1488,11 @@ __device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask
                 sgm_step<2, false>(S.SN, C, S.mn, P1, P2, i == 0 ? S.edge_lo : 0xFFFFFFFFu,
                                    i == 0 ? S.edge_hi : 0xFFFFFFFFu);
                 __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(S.SN[0] | (S.SN[1] << 8)), rn, real ? so_n : kDrop, 0, 0);
-                if (CHECK && q < q0) {
-                    S.SN[0] = 0;
-                    S.SN[1] = 0;
-                    S.mn = 0;
+                if constexpr (CHECK) {
+                    const bool pre = q < q0;
+                    S.SN[0] = pre ? 0u : S.SN[0];
+                    S.SN[1] = pre ? 0u : S.SN[1];
+                    S.mn = pre ? 0u : S.mn;
                 }
             } else {
                 sgm_step<NB, false>(S.SF, C, S.mn, P1, P2);
@@ -1467,11 +1504,12 @@ __device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask
                     }
                 }
                 st_bytes<NB, (NB >= 8)>(ro, real ? so_f : kDrop, S.SF);
-                if (CHECK && q < q0) {
+                if constexpr (CHECK) {
+                    const bool pre = q < q0;
 #pragma unroll
                     for (int j = 0; j < NB; j++)
-                        S.SF[j] = 0;
-                    S.mn = 0;
+                        S.SF[j] = pre ? 0u : S.SF[j];
+                    S.mn = pre ? 0u : S.mn;
                 }
             }
             so_n += sstep_n;
@@ -1484,26 +1522,46 @@ __device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask
 
     // x >= D - 1 on every line of the wave: no disparity is ever invalid
     const bool all_valid = grp * kLinesPerWave >= D - 1;
-#pragma nounroll
-    for (int k = 0; k < nt; k++) {
+    auto variant = [&](int k) {
+        const bool check = 16 * k < q0 || 16 * k + 16 > qend || !all_valid || cur.mode == 2;
+        return (cur.mode == 0 ? 0 : 2) + (check ? 1 : 0);
+    };
+    auto one = [&](int k, auto v_tag) {
+        constexpr int V = decltype(v_tag)::value;
         if (k > 0 && win_changes<NB>(prev, cur))
             S.exchange(xs + g * D, l, P1, prev, cur);
-        const bool check = 16 * k < q0 || 16 * k + 16 > qend || !all_valid || cur.mode == 2;
-        if (cur.mode == 0) {
-            if (check)
-                segment(k, std::integral_constant<int, 0>{}, std::true_type{});
-            else
-                segment(k, std::integral_constant<int, 0>{}, std::false_type{});
-        } else {
-            if (check)
-                segment(k, std::integral_constant<int, 1>{}, std::true_type{});
-            else
-                segment(k, std::integral_constant<int, 1>{}, std::false_type{});
-        }
+        segment(k, std::integral_constant<int, (V >> 1)>{}, std::integral_constant<bool, (V & 1) != 0>{});
         prev = cur;
         cur = nxt;
         nxt = win_seg<NB>(__builtin_amdgcn_readfirstlane(raw_nn));
         raw_nn = seg_raw(k + 3);
+    };
+    // maximal runs of one variant, one loop each (see path_window_rows)
+    int k = 0;
+#pragma nounroll
+    while (k < nt) {
+        const int v = variant(k);
+        if (v == 0) {
+#pragma nounroll
+            do
+                one(k++, std::integral_constant<int, 0>{});
+            while (k < nt && variant(k) == 0);
+        } else if (v == 1) {
+#pragma nounroll
+            do
+                one(k++, std::integral_constant<int, 1>{});
+            while (k < nt && variant(k) == 1);
+        } else if (v == 2) {
+#pragma nounroll
+            do
+                one(k++, std::integral_constant<int, 2>{});
+            while (k < nt && variant(k) == 2);
+        } else {
+#pragma nounroll
+            do
+                one(k++, std::integral_constant<int, 3>{});
+            while (k < nt && variant(k) == 3);
+        }
     }
 }
 
