@@ -77,11 +77,16 @@ int sgmh_get_times(sgmh_ctx *ctx, sgmh_times *t);
  *   SGMH_LAYOUT_LINES4 four scanlines per wave, 16 lanes each, D/16
  *                      disparities per lane;
  *   SGMH_LAYOUT_WAVE   one scanline per wave, D/64 disparities per lane
- *                      (D = 64, 128 or 256; other D fail at the call).
+ *                      (D = 64, 128 or 256; other D fail at the call);
+ *   SGMH_LAYOUT_PLACED four scanlines per wave, one workgroup per compute
+ *                      unit whose waves take row or column line groups by
+ *                      the SIMD they run on (4 paths, D = 32, 64 or 128;
+ *                      other cases take the four-line launch).
  * Returns -1 for an unknown layout. */
 #define SGMH_LAYOUT_AUTO 0
 #define SGMH_LAYOUT_LINES4 1
 #define SGMH_LAYOUT_WAVE 2
+#define SGMH_LAYOUT_PLACED 3
 int sgmh_set_path_layout(sgmh_ctx *ctx, int layout);
 
 /* ---- plain SGM: replaces the body of sgm_run_fn (common/sgm.h:29-31) ---- */

@@ -161,6 +161,84 @@ __global__ __launch_bounds__(64) void k_path(PathArgs a)
     path_block<DPL, FROM_CENSUS, SAT, DECIM, BCAST, LPL>(a, (int)blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// Placed plain paths (census-fed, 4 directions, four lines per wave).
+//
+// A row line group is 1920 steps at 1080p against a column group's 1080, and
+// a SIMD issues one wave64 VALU instruction per 4 cycles whichever wave it
+// comes from.  The hardware places the one-wave workgroups of k_path by wave
+// count (about 1.5 per SIMD at 1080p), so about half the SIMDs hold a row wave
+// and a column wave: 0.28 ms of issue at 1080p D=64 against the 0.19 ms a
+// balanced placement would need, and the launch takes 0.26 ms.  Here one
+// 8-wave workgroup holds each CU (its LDS reservation keeps a second one off
+// it), and each wave reads which SIMD it runs on: on the CU's first two SIMDs
+// (three on the first `extra` CUs) the first wave takes one row group and the
+// other waves leave; on the other SIMDs two waves take column groups.  Groups
+// come from two queues (rows, columns); every wave that has done its share
+// drains what is left of both, so the result does not depend on placement.
+// The last wave out resets the queue counters for the next launch.
+constexpr int kPlacedThreads = 512;          // 8 waves: two per SIMD
+constexpr int kPlacedLds = 81 * 1024;        // dynamic LDS: a second one would not fit in 160 KB
+constexpr int kQueueStride = 16;             // ints: one 64-byte line per counter
+
+struct PlacedQ {
+    int kr[2], kc[2]; // direction-table indices of the row and the column directions
+    int gr, gc;       // line groups per row / column direction
+    int nrow, ncol;   // row / column groups in all
+    int extra;        // CUs with three row SIMDs
+    int total_waves;  // waves in the launch
+};
+
+template <int DPL, bool BCAST>
+__global__ __launch_bounds__(kPlacedThreads) void k_path_placed(PathArgs a, PlacedQ q, int *__restrict__ ctr)
+{
+    __shared__ int simd_waves[4];
+    if (threadIdx.x < 4)
+        simd_waves[threadIdx.x] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    // hwreg(HW_REG_HW_ID): SIMD_ID is bits 5:4
+    const int simd = (int)((__builtin_amdgcn_s_getreg((31 << 11) | 4) >> 4) & 3);
+    int slot = 0;
+    if (lane == 0)
+        slot = atomicAdd(&simd_waves[simd], 1);
+    slot = __builtin_amdgcn_readfirstlane(__shfl(slot, 0));
+    const int row_simds = 2 + ((int)blockIdx.x < q.extra ? 1 : 0);
+    const bool row_role = simd < row_simds && slot == 0;
+    const bool col_role = simd >= row_simds && slot < 2;
+    // phases: 0 the row role's one row group, 1 column groups until none is
+    // left, 2 row groups until none is left, 3 done.  One call site, so the
+    // path code inlines.
+    int phase = row_role ? 0 : (col_role ? 1 : 3);
+    while (phase < 3) {
+        const int which = phase == 1 ? 1 : 0;
+        int it = 0;
+        if (lane == 0)
+            it = atomicAdd(ctr + which * kQueueStride, 1);
+        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+        if (it >= (which == 0 ? q.nrow : q.ncol)) {
+            phase++;
+            continue;
+        }
+        if (phase == 0)
+            phase = 1;
+        const int g = which == 0 ? q.gr : q.gc, h = it >= g ? 1 : 0;
+        const int k = which == 0 ? q.kr[h] : q.kc[h];
+        path_group<DPL, true, false, false, PathArgs, BCAST>(a, a.dir[k], it - h * g);
+    }
+    // Every wave counts itself out after its last pop (release: the pops are
+    // done first); the last one resets the three counters, which no other wave
+    // of this launch touches any more.
+    if (lane == 0) {
+        const int done = __hip_atomic_fetch_add(ctr + 2 * kQueueStride, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (done == q.total_waves - 1) {
+            __hip_atomic_store(ctr, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(ctr + kQueueStride, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(ctr + 2 * kQueueStride, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // The two multiscale stages' 8-path aggregations in one launch: they read
 // different cost volumes (C2 already holds the merge with C1) and write
 // different volumes, so nothing orders them.  Blocks [0, nhalf) run the
@@ -450,6 +528,62 @@ static bool wave_lines_pay(int W, int H, int D, int ndirs)
     return D == 128 && (long)W * H <= kWaveLinesMaxPixelsD128;
 }
 
+// The placed launch pays where the four-line groups outnumber the SIMDs (so
+// the one-wave-per-workgroup launch doubles some of them up) and a row step
+// is long (D = 128).  Measured at 1920x1080, aggregate ms, four-line / placed:
+// D = 128 0.477 / 0.434; D = 64 0.269 / 0.279 (the column SIMDs, two column
+// waves each, then finish last); D = 32 0.166 / 0.221; 1280x720 D = 64 0.176
+// / 0.202.  With two frames in flight its CU-wide LDS reservation also keeps
+// the other frame's kernels off the CU: D = 64 0.317 / 0.380 ms per frame.
+static bool placed_pays(sgmh_ctx *ctx, int W, int H, int D, int ndirs)
+{
+    if (!ctx || ndirs != 4 || ctx->cus <= 0 || D != 128)
+        return false;
+    const long groups = 2L * ((H + kLinesPerWave - 1) / kLinesPerWave) + 2L * ((W + kLinesPerWave - 1) / kLinesPerWave);
+    return groups > 4L * ctx->cus && 2L * ((H + kLinesPerWave - 1) / kLinesPerWave) <= 3L * ctx->cus;
+}
+
+static int launch_paths_placed(sgmh_ctx *ctx, hipStream_t s, PathArgs a, const int (*dirs)[3], uint8_t *vol,
+                               size_t dir_stride)
+{
+    if (fill_paths(a, dirs, 4, true, false, vol, dir_stride, 0, false) < 0)
+        return -1;
+    if (ensure(ctx->queues, 3 * kQueueStride * sizeof(int)))
+        return -1;
+    static_assert(sizeof(int) * 3 * kQueueStride <= 256, "queue counters");
+    PlacedQ q{};
+    int nr = 0, nc = 0;
+    for (int k = 0; k < a.ndirs; k++) {
+        if (a.dir[k].dy == 0)
+            q.kr[nr++] = k;
+        else
+            q.kc[nc++] = k;
+    }
+    REQUIRE(nr == 2 && nc == 2, "sgmh: the placed path launch needs two row and two column directions");
+    q.gr = (a.H + kLinesPerWave - 1) / kLinesPerWave;
+    q.gc = (a.W + kLinesPerWave - 1) / kLinesPerWave;
+    q.nrow = 2 * q.gr;
+    q.ncol = 2 * q.gc;
+    q.extra = std::max(0, std::min(ctx->cus, q.nrow - 2 * ctx->cus));
+    q.total_waves = ctx->cus * (kPlacedThreads / 64);
+    int *ctr = (int *)ctx->queues.p;
+    const dim3 grid(ctx->cus), block(kPlacedThreads);
+    const bool bcast = a.H >= kRowBcastMinH && a.D == 64;
+    switch (a.D / 16) {
+    case 2: hipLaunchKernelGGL((k_path_placed<2, false>), grid, block, kPlacedLds, s, a, q, ctr); break;
+    case 4:
+        if (bcast)
+            hipLaunchKernelGGL((k_path_placed<4, true>), grid, block, kPlacedLds, s, a, q, ctr);
+        else
+            hipLaunchKernelGGL((k_path_placed<4, false>), grid, block, kPlacedLds, s, a, q, ctr);
+        break;
+    case 8: hipLaunchKernelGGL((k_path_placed<8, false>), grid, block, kPlacedLds, s, a, q, ctr); break;
+    default: set_err("sgmh: the placed path launch takes D = 32, 64 or 128 (D = %d)", a.D); return -1;
+    }
+    CK(hipGetLastError());
+    return 0;
+}
+
 static int plain_paths(sgmh_ctx *ctx, hipStream_t s, const uint64_t *cl, const uint64_t *cr, int W,
                        int H, int D, const int (*dirs)[3], int ndirs, uint8_t *vol, size_t dir_stride)
 {
@@ -463,6 +597,10 @@ static int plain_paths(sgmh_ctx *ctx, hipStream_t s, const uint64_t *cl, const u
     a.D = D;
     a.P2 = kPlainP2;
     a.invalid = kPlainInvalid;
+    const bool placed = ctx && ndirs == 4 && (D == 32 || D == 64 || D == 128) && layout != SGMH_LAYOUT_WAVE &&
+                        (layout == SGMH_LAYOUT_PLACED || (layout == SGMH_LAYOUT_AUTO && !wave && placed_pays(ctx, W, H, D, ndirs)));
+    if (placed)
+        return launch_paths_placed(ctx, s, a, dirs, vol, dir_stride);
     return launch_paths(s, a, dirs, ndirs, true, false, vol, dir_stride, 0, wave);
 }
 
@@ -739,6 +877,14 @@ int sgmh_ctx_create(sgmh_ctx **out, int device)
         delete c;
         return -1;
     }
+    // the placed path launch's queue counters start at zero; each launch's
+    // last wave resets them
+    if (hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        ensure(c->queues, 256) || hipMemset(c->queues.p, 0, 256) != hipSuccess) {
+        set_err("sgmh_ctx_create: queue counters");
+        sgmh_ctx_destroy(c);
+        return -1;
+    }
     *out = c;
     return 0;
 }
@@ -750,7 +896,7 @@ void sgmh_ctx_destroy(sgmh_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     DevBuf *bufs[] = {&c->img_l, &c->img_r, &c->disp, &c->disp1, &c->cl, &c->cr, &c->vol,
-                      &c->cl1, &c->cr1, &c->cost1, &c->cost2, &c->vol1};
+                      &c->cl1, &c->cr1, &c->cost1, &c->cost2, &c->vol1, &c->queues};
     for (DevBuf *b : bufs)
         free_buf(*b);
     for (hipEvent_t e : c->events)
@@ -770,7 +916,8 @@ int sgmh_set_timing(sgmh_ctx *c, int on)
 int sgmh_set_path_layout(sgmh_ctx *c, int layout)
 {
     REQUIRE(c, "sgmh_set_path_layout: NULL context");
-    REQUIRE(layout == SGMH_LAYOUT_AUTO || layout == SGMH_LAYOUT_LINES4 || layout == SGMH_LAYOUT_WAVE,
+    REQUIRE(layout == SGMH_LAYOUT_AUTO || layout == SGMH_LAYOUT_LINES4 || layout == SGMH_LAYOUT_WAVE ||
+                layout == SGMH_LAYOUT_PLACED,
             "sgmh_set_path_layout: unknown layout %d", layout);
     c->path_layout = layout;
     return 0;

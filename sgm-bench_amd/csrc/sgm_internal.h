@@ -52,6 +52,8 @@ struct sgmh_ctx {
     DevBuf cl, cr;                             // census codes
     DevBuf vol;                                // direction volumes
     DevBuf cl1, cr1, cost1, cost2, vol1;       // multiscale stage buffers
+    DevBuf queues;                             // work-queue counters of the placed path launch (zeroed once)
+    int cus = 0;                               // compute units of the device
 };
 
 inline int ensure(DevBuf &b, size_t bytes)

@@ -86,7 +86,7 @@ class Pair(ctypes.Structure):
 
 PLAIN, MULTISCALE = 0, 1
 BATCH_AUTOTUNE, BATCH_NO_GRAPH, BATCH_CENSUS_32X32, BATCH_XCD_SLICES = 1, 2, 4, 8
-LAYOUT_AUTO, LAYOUT_LINES4, LAYOUT_WAVE = 0, 1, 2  # SGMH_LAYOUT_* (include/sgm_hip.h)
+LAYOUT_AUTO, LAYOUT_LINES4, LAYOUT_WAVE, LAYOUT_PLACED = 0, 1, 2, 3  # SGMH_LAYOUT_* (include/sgm_hip.h)
 
 
 def hip_lib() -> ctypes.CDLL:

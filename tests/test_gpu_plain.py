@@ -47,6 +47,10 @@ def layout_ctx(ctx, request):
 
 
 LAYOUTS = [pytest.param(sgm_amd.LAYOUT_LINES4, id="lines4"), pytest.param(sgm_amd.LAYOUT_WAVE, id="wave")]
+# the placed launch (one workgroup per CU, waves take row or column groups by
+# their SIMD, queues reset by the last wave) runs 4-path frames at D = 32, 64
+# and 128; the others fall back to the four-line launch
+LAYOUTS_ALL = LAYOUTS + [pytest.param(sgm_amd.LAYOUT_PLACED, id="placed")]
 
 
 def check_path_volumes(ctx, D, H, W):
@@ -105,7 +109,7 @@ def test_plain_matches_oracle(ctx, H, W, D, paths):
     assert bad.size == 0, f"{bad.shape[0]} pixels differ, first at {bad[0]}"
 
 
-@pytest.mark.parametrize("layout_ctx", LAYOUTS, indirect=True)
+@pytest.mark.parametrize("layout_ctx", LAYOUTS_ALL, indirect=True)
 @pytest.mark.parametrize("H,W,D", [(61, 97, 64), (300, 40, 64), (120, 200, 128), (20, 1000, 256),
                                    (515, 77, 64), (480, 640, 64)])
 def test_plain_layouts_match_oracle(layout_ctx, H, W, D):
@@ -147,7 +151,7 @@ def test_computed_goldens_plain(ctx, goldens):
         assert sgm_amd.fnv1a64(out) == g["fnv"], g["name"]
 
 
-@pytest.mark.parametrize("layout_ctx", LAYOUTS, indirect=True)
+@pytest.mark.parametrize("layout_ctx", LAYOUTS_ALL, indirect=True)
 def test_goldens_both_layouts(layout_ctx, goldens):
     """Every plain golden with D = 64, 128 or 256 (the reference's on-disk ones
     and the compiled reference's hashes, 4 and 8 paths, up to 1920x1080) with
