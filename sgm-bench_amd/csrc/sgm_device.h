@@ -1046,6 +1046,11 @@ constexpr uint32_t kWinAbsent = 255u;
 constexpr int kWinMaxTiles = 1024;      // tiles along a line (W, H <= 16384)
 constexpr uint32_t kWinPenalty = 1000u; // cost of an absent disparity inside a step: L > 255
 
+// LDS one wave of the windowed kernels uses: the layout exchange row of its
+// four lines and its line's tile ranges
+template <int NB>
+constexpr int win_wave_lds() { return kLinesPerWave * 16 * NB + kWinMaxTiles; }
+
 struct WinArgs {
     const uint64_t *cl, *cr; // census codes, pixel i at index kLead + i
     const uint8_t *blk;      // per-tile block ranges
@@ -1157,13 +1162,14 @@ __device__ __forceinline__ bool win_changes(const WinSeg &o, const WinSeg &c)
 // ahead), so every step of every mode issues the same loads in the same
 // order and the prefetch ring's counters are exact.
 template <int NB, int DIR>
-__device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask &t, uint8_t *narrow, int grp)
+__device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask &t, uint8_t *narrow, int grp,
+                                                 uint8_t *wlds)
 {
     constexpr int PF = 8;
     constexpr int D = 16 * NB;
     constexpr int NC = NB / 2;
     typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-    __shared__ __align__(16) uint8_t xs[kLinesPerWave * D]; // layout exchange (one wave per workgroup)
+    uint8_t *xs = wlds;                           // layout exchange, kLinesPerWave * D bytes of this wave's LDS
     __builtin_amdgcn_s_setprio(2); // rows are the frame's longest lines: its critical path
     const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
     const int W = a.W, H = a.H;
@@ -1181,10 +1187,14 @@ __device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask
     // The line's tile ranges in traversal order, staged in LDS once: a segment
     // reads its range a few segments ahead from there, so no wait for a global
     // load (which would drain the code prefetch ring) falls inside the line.
-    __shared__ uint8_t segs[kWinMaxTiles];
+    uint8_t *segs = wlds + kLinesPerWave * D; // kWinMaxTiles bytes of this wave's LDS
     for (int k = lane; k < nt; k += 64)
         segs[k] = a.blk[tile_y * a.tiles_x + (DIR > 0 ? k : nt - 1 - k)];
-    __syncthreads(); // one wave per workgroup: orders the staging before the reads
+    // this wave alone reads what it staged, and one wave's LDS accesses run in
+    // order: keeping the compiler from moving the reads above suffices
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     auto seg_raw = [&](int k) -> uint32_t { return segs[k < nt ? k : nt - 1]; }; // the same in every lane
     auto half_code = [](const v4u &v, int half) -> uint64_t {
         return half ? ((uint64_t)v[3] << 32) | v[2] : ((uint64_t)v[1] << 32) | v[0];
@@ -1389,13 +1399,14 @@ __device__ __forceinline__ void path_window_rows(const WinArgs &a, const DirTask
 // x - d < 0 is fixed too: folded, with the MASKED penalties, into one
 // per-segment floor under the cost.
 template <int NB>
-__device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask &t, uint8_t *narrow, int grp)
+__device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask &t, uint8_t *narrow, int grp,
+                                                 uint8_t *wlds)
 {
     constexpr int PF = 4;
     constexpr int D = 16 * NB;
     constexpr int NC = NB / 2;
     typedef uint32_t v4u __attribute__((ext_vector_type(4)));
-    __shared__ __align__(16) uint8_t xs[kLinesPerWave * D]; // layout exchange (one wave per workgroup)
+    uint8_t *xs = wlds; // layout exchange, kLinesPerWave * D bytes of this wave's LDS
     const int lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
     const int W = a.W, H = a.H;
     const int dir = t.dy;
@@ -1409,10 +1420,12 @@ __device__ __forceinline__ void path_window_cols(const WinArgs &a, const DirTask
     const uint32_t P1 = (uint32_t)t.P1, P2 = kPlainP2, inval = kPlainInvalid;
     const int q0 = dir > 0 ? 0 : 16 * nt - H, qend = dir > 0 ? H : 16 * nt;
     // the line's tile ranges in traversal order, staged in LDS once (as for rows)
-    __shared__ uint8_t segs[kWinMaxTiles];
+    uint8_t *segs = wlds + kLinesPerWave * D; // kWinMaxTiles bytes of this wave's LDS
     for (int k = lane; k < nt; k += 64)
         segs[k] = a.blk[(dir > 0 ? k : nt - 1 - k) * a.tiles_x + tile_x];
-    __syncthreads(); // one wave per workgroup: orders the staging before the reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); // as in path_window_rows
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     auto seg_raw = [&](int k) -> uint32_t { return segs[k < nt ? k : nt - 1]; }; // the same in every lane
     auto half_code = [](const v4u &v, int half) -> uint64_t {
         return half ? ((uint64_t)v[3] << 32) | v[2] : ((uint64_t)v[1] << 32) | v[0];

@@ -189,8 +189,12 @@ struct PlacedQ {
     int total_waves;  // waves in the launch
 };
 
-template <int DPL, bool BCAST>
-__global__ __launch_bounds__(kPlacedThreads) void k_path_placed(PathArgs a, PlacedQ q, int *__restrict__ ctr)
+// The placed launch's wave loop: role from the SIMD, then groups from the two
+// queues (run(k, grp) takes line group grp of direction-table entry k), then
+// the count-out and the reset.  `run` is called from one site, so the path
+// code it holds inlines.
+template <class Run>
+__device__ __forceinline__ void placed_loop(const PlacedQ &q, int *__restrict__ ctr, Run &&run)
 {
     __shared__ int simd_waves[4];
     if (threadIdx.x < 4)
@@ -207,8 +211,7 @@ __global__ __launch_bounds__(kPlacedThreads) void k_path_placed(PathArgs a, Plac
     const bool row_role = simd < row_simds && slot == 0;
     const bool col_role = simd >= row_simds && slot < 2;
     // phases: 0 the row role's one row group, 1 column groups until none is
-    // left, 2 row groups until none is left, 3 done.  One call site, so the
-    // path code inlines.
+    // left, 2 row groups until none is left, 3 done
     int phase = row_role ? 0 : (col_role ? 1 : 3);
     while (phase < 3) {
         const int which = phase == 1 ? 1 : 0;
@@ -223,8 +226,7 @@ __global__ __launch_bounds__(kPlacedThreads) void k_path_placed(PathArgs a, Plac
         if (phase == 0)
             phase = 1;
         const int g = which == 0 ? q.gr : q.gc, h = it >= g ? 1 : 0;
-        const int k = which == 0 ? q.kr[h] : q.kc[h];
-        path_group<DPL, true, false, false, PathArgs, BCAST>(a, a.dir[k], it - h * g);
+        run(which == 0 ? q.kr[h] : q.kc[h], it - h * g);
     }
     // Every wave counts itself out after its last pop (release: the pops are
     // done first); the last one resets the three counters, which no other wave
@@ -237,6 +239,12 @@ __global__ __launch_bounds__(kPlacedThreads) void k_path_placed(PathArgs a, Plac
             __hip_atomic_store(ctr + 2 * kQueueStride, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+}
+
+template <int DPL, bool BCAST>
+__global__ __launch_bounds__(kPlacedThreads) void k_path_placed(PathArgs a, PlacedQ q, int *__restrict__ ctr)
+{
+    placed_loop(q, ctr, [&](int k, int grp) { path_group<DPL, true, false, false, PathArgs, BCAST>(a, a.dir[k], grp); });
 }
 
 // The two multiscale stages' 8-path aggregations in one launch: they read
@@ -271,21 +279,27 @@ __global__ __launch_bounds__(256) void k_wta(const uint8_t *__restrict__ L, size
 // windowed SGM: every direction of the frame in one launch, one line group
 // (4 rows or 4 columns) per wave
 template <int NB>
+__device__ __forceinline__ void window_group(const WinArgs &a, int k, int grp, uint8_t *wlds)
+{
+    const DirTask &t = a.dir[k];
+    if (t.dy == 0) {
+        if (t.dx > 0)
+            path_window_rows<NB, 1>(a, t, a.Ln[k], grp, wlds);
+        else
+            path_window_rows<NB, -1>(a, t, a.Ln[k], grp, wlds);
+    } else {
+        path_window_cols<NB>(a, t, a.Ln[k], grp, wlds);
+    }
+}
+
+template <int NB>
 __global__ __launch_bounds__(64) void k_path_window(WinArgs a)
 {
+    __shared__ __align__(16) uint8_t wlds[win_wave_lds<NB>()];
     int k = 0;
     while (k + 1 < a.ndirs && (int)blockIdx.x >= a.dir[k + 1].first_block)
         k++;
-    const DirTask t = a.dir[k];
-    const int grp = (int)blockIdx.x - t.first_block;
-    if (t.dy == 0) {
-        if (t.dx > 0)
-            path_window_rows<NB, 1>(a, t, a.Ln[k], grp);
-        else
-            path_window_rows<NB, -1>(a, t, a.Ln[k], grp);
-    } else {
-        path_window_cols<NB>(a, t, a.Ln[k], grp);
-    }
+    window_group<NB>(a, k, (int)blockIdx.x - a.dir[k].first_block, wlds);
 }
 
 template <int NB>
@@ -543,29 +557,37 @@ static bool placed_pays(sgmh_ctx *ctx, int W, int H, int D, int ndirs)
     return groups > 4L * ctx->cus && 2L * ((H + kLinesPerWave - 1) / kLinesPerWave) <= 3L * ctx->cus;
 }
 
+// The placed launches' queues over a 4-direction table (two row and two
+// column directions).
+static_assert(sizeof(int) * 3 * kQueueStride <= 256, "queue counters");
+static int placed_queues(const sgmh_ctx *ctx, const DirTask *dir, int ndirs, int W, int H, PlacedQ &q)
+{
+    q = PlacedQ{};
+    int nr = 0, nc = 0;
+    for (int k = 0; k < ndirs; k++) {
+        if (dir[k].dy == 0 && nr < 2)
+            q.kr[nr++] = k;
+        else if (dir[k].dx == 0 && nc < 2)
+            q.kc[nc++] = k;
+    }
+    REQUIRE(ndirs == 4 && nr == 2 && nc == 2, "sgmh: the placed launch needs two row and two column directions");
+    q.gr = (H + kLinesPerWave - 1) / kLinesPerWave;
+    q.gc = (W + kLinesPerWave - 1) / kLinesPerWave;
+    q.nrow = 2 * q.gr;
+    q.ncol = 2 * q.gc;
+    q.extra = std::max(0, std::min(ctx->cus, q.nrow - 2 * ctx->cus));
+    q.total_waves = ctx->cus * (kPlacedThreads / 64);
+    return 0;
+}
+
 static int launch_paths_placed(sgmh_ctx *ctx, hipStream_t s, PathArgs a, const int (*dirs)[3], uint8_t *vol,
                                size_t dir_stride)
 {
     if (fill_paths(a, dirs, 4, true, false, vol, dir_stride, 0, false) < 0)
         return -1;
-    if (ensure(ctx->queues, 3 * kQueueStride * sizeof(int)))
+    PlacedQ q;
+    if (placed_queues(ctx, a.dir, a.ndirs, a.W, a.H, q))
         return -1;
-    static_assert(sizeof(int) * 3 * kQueueStride <= 256, "queue counters");
-    PlacedQ q{};
-    int nr = 0, nc = 0;
-    for (int k = 0; k < a.ndirs; k++) {
-        if (a.dir[k].dy == 0)
-            q.kr[nr++] = k;
-        else
-            q.kc[nc++] = k;
-    }
-    REQUIRE(nr == 2 && nc == 2, "sgmh: the placed path launch needs two row and two column directions");
-    q.gr = (a.H + kLinesPerWave - 1) / kLinesPerWave;
-    q.gc = (a.W + kLinesPerWave - 1) / kLinesPerWave;
-    q.nrow = 2 * q.gr;
-    q.ncol = 2 * q.gc;
-    q.extra = std::max(0, std::min(ctx->cus, q.nrow - 2 * ctx->cus));
-    q.total_waves = ctx->cus * (kPlacedThreads / 64);
     int *ctr = (int *)ctx->queues.p;
     const dim3 grid(ctx->cus), block(kPlacedThreads);
     const bool bcast = a.H >= kRowBcastMinH && a.D == 64;
@@ -773,6 +795,12 @@ int window_rest(sgmh_ctx *ctx, hipStream_t s, int W, int H, int D, const uint8_t
         a.Ln[k] = nvol + (size_t)k * N * 32;
         blocks += (t.nlines + kLinesPerWave - 1) / kLinesPerWave;
     }
+    // One wave per workgroup.  A placed launch as for the plain D=128 paths
+    // (rows alone on their SIMDs, two column waves on each other SIMD) was
+    // measured slower here: aggregate ms at 1080p, one-wave / placed, the
+    // network's proposal 0.333 / 0.377, every tile NARROW 0.191 / 0.251, every
+    // tile FULL 0.546 / 0.503.  Two column waves on one SIMD issue at
+    // about 1.65x one alone, and the NARROW steps are short.
     {
         PhaseScope ps(ctx, s, PH_AGG);
         switch (D / 16) {
