@@ -216,11 +216,14 @@ typedef struct sgmh_batch sgmh_batch;
  * otherwise makes: */
 #define SGMH_BATCH_CENSUS_32X32 4 /* census in 32x32 tiles instead of 64x16 */
 #define SGMH_BATCH_XCD_SLICES 8   /* path line groups taken from per-XCD slices of the queue */
+#define SGMH_BATCH_FUSED 16       /* path aggregation and sum + argmin in one launch, each pair's
+                                     argmin as soon as its directions are done (ignores XCD_SLICES) */
 
 /* Plan a batch: allocate every pair's workspace, build the work lists of the
  * persistent per-phase kernels (pre-filter / census, cost, path aggregation,
  * sum + argmin), optionally autotune the schedule (lane partition, path item
- * order, path waves per SIMD, census tile shape, scanline grouping), and
+ * order, path waves per SIMD, census tile shape, scanline grouping, separate
+ * or fused path and argmin launches), and
  * capture the phases as one hipGraph.  The device buffers named in `pairs` are bound into the plan:
  * later runs read whatever they then hold. */
 int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgmh_batch **out);

@@ -16,6 +16,8 @@
 // The launches are captured once into a hipGraph; a run is one graph launch.
 // The autotuner times candidate schedules (persistent waves per SIMD for the
 // path phase, item order) on the batch itself and keeps the fastest.
+// SGMH_BATCH_FUSED instead runs path and argmin in one launch, k_batch_fused,
+// which starts each pair's argmin as soon as its directions are complete.
 //
 // Every pair is computed by the same device functions as the single-pair
 // entry points (sgm_device.h), so each stays bit-exact to its oracle.
@@ -64,6 +66,7 @@ struct BatchPathTask {
     uint32_t out_bytes;
     DirTask dir;
     int variant; // kVar* below
+    int dep;     // the sum + argmin task this direction feeds
 };
 
 enum {
@@ -78,6 +81,7 @@ struct WtaTask {
     int ndirs, npix, D;
     uint8_t *disp;
     int first_item;
+    int units; // fused launch: claimable units of kWtaUnit wta_batch calls
 };
 
 // The items of a launch are split into contiguous chunks, one per block (or
@@ -254,6 +258,16 @@ __global__ __launch_bounds__(64) void k_batch_path_xcd(const BatchPathTask *__re
     }
 }
 
+// The lane index through an empty asm statement: lane arithmetic derived from
+// it is recomputed where it is used rather than hoisted out of a persistent
+// loop, where it would hold registers across everything else the loop runs.
+__device__ __forceinline__ int opaque_lane()
+{
+    int l = threadIdx.x & 63;
+    asm volatile("" : "+v"(l));
+    return l;
+}
+
 // one item = one wta_batch of one task (64/LPP * kWtaBatch pixels); each wave
 // takes a contiguous chunk of items
 __global__ __launch_bounds__(256) void k_batch_wta(const WtaTask *__restrict__ tasks, int ntasks, int nitems)
@@ -271,14 +285,154 @@ __global__ __launch_bounds__(256) void k_batch_wta(const WtaTask *__restrict__ t
         const int lpp = tk.D / 16;
         const int b0 = (item - tk.first_item) * (64 / lpp) * kWtaBatch;
         const bool eight = tk.ndirs == 8;
+        const int lane = opaque_lane();
         switch (lpp) {
-        case 2: eight ? wta_batch<2, 8>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0)
-                      : wta_batch<2, 4>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0); break;
-        case 4: eight ? wta_batch<4, 8>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0)
-                      : wta_batch<4, 4>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0); break;
-        default: eight ? wta_batch<8, 8>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0)
-                       : wta_batch<8, 4>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0); break;
+        case 2: eight ? wta_batch<2, 8>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0, lane)
+                      : wta_batch<2, 4>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0, lane); break;
+        case 4: eight ? wta_batch<4, 8>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0, lane)
+                      : wta_batch<4, 4>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0, lane); break;
+        default: eight ? wta_batch<8, 8>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0, lane)
+                       : wta_batch<8, 4>(tk.L, tk.dir_stride, tk.npix, tk.disp, b0, lane); break;
         }
+    }
+}
+
+// ---- path aggregation and sum + argmin in one persistent launch ----
+//
+// A sum + argmin task (one plain pair, or one multiscale stage) depends on the
+// path items of its directions only.  In the fused launch a wave that
+// completes a path item counts its task's remaining items down (a release at
+// device scope: its volume stores are visible before the count); the wave
+// that takes the count to zero publishes the task in the next ready slot.
+// Waves then take argmin units from ready slots, so the memory-bound argmin
+// of early tasks overlaps the arithmetic-bound aggregation of later ones
+// instead of running after all of it.  Waves whose block index modulo 4 is
+// below `wta_quarters` look for argmin units first, the others for path
+// items first; both take the other kind when theirs has run out, and a wave
+// with neither sleeps briefly while path items are still in flight.  Every
+// claimed path item is held by a running wave, so every task is published
+// and the launch drains.
+//
+// Synchronisation block (ints, one 64-byte line per counter):
+//   [0] path item head, [1] publish counter,
+//   [2 + t] remaining path items of task t,
+//   [2 + nwta + s] 1 + the task published in slot s (0 = not yet),
+//   [2 + 2 nwta + s] argmin units claimed from slot s.
+constexpr int kWtaUnit = 16; // wta_batch calls per claimed argmin unit
+// pixel batches per wta_batch call in the fused launch: eight-direction tasks
+// load half as many at a time, so every task keeps 256 bytes per lane in
+// flight and the kernel stays within 168 registers (three waves per SIMD)
+template <int NDIRS>
+constexpr int kFusedWtaBatch = NDIRS == 8 ? kWtaBatch / 2 : kWtaBatch;
+
+__device__ __forceinline__ int wave_fetch_add(int *p, int v)
+{
+    int r = 0;
+    if (threadIdx.x == 0)
+        r = atomicAdd(p, v);
+    return __builtin_amdgcn_readfirstlane(__shfl(r, 0));
+}
+
+template <int LPP, int NDIRS>
+__device__ __forceinline__ void wta_unit(const WtaTask &tk, int u)
+{
+    constexpr int U = kFusedWtaBatch<NDIRS>, per = (64 / LPP) * U; // pixels per wta_batch call
+    const int nb = (tk.npix + per - 1) / per;
+    const int j1 = min(nb, (u + 1) * kWtaUnit);
+    for (int j = u * kWtaUnit; j < j1; j++)
+        wta_batch<LPP, NDIRS, U>(tk.L, tk.dir_stride, tk.npix, tk.disp, j * per, opaque_lane());
+}
+
+__device__ __forceinline__ void fused_wta_unit(const WtaTask &tk, int u)
+{
+    const bool eight = tk.ndirs == 8;
+    switch (tk.D) {
+    case 32: eight ? wta_unit<2, 8>(tk, u) : wta_unit<2, 4>(tk, u); break;
+    case 64: eight ? wta_unit<4, 8>(tk, u) : wta_unit<4, 4>(tk, u); break;
+    default: eight ? wta_unit<8, 8>(tk, u) : wta_unit<8, 4>(tk, u); break;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fused_reset(int *__restrict__ sync, const int *__restrict__ left0, int nwta)
+{
+    const int n = 2 + 3 * nwta;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        sync[i * kCounterStride] = (i >= 2 && i < 2 + nwta) ? left0[i - 2] : 0;
+}
+
+__global__ __launch_bounds__(64) void k_batch_fused(const BatchPathTask *__restrict__ tasks,
+                                                    const uint32_t *__restrict__ items, int nitems,
+                                                    const WtaTask *__restrict__ wtas, int nwta,
+                                                    int *__restrict__ sync, int wta_quarters)
+{
+    int *head = sync, *pub = sync + kCounterStride;
+    int *left = sync + 2 * kCounterStride;
+    int *ready = left + nwta * kCounterStride, *taken = ready + nwta * kCounterStride;
+    const bool wta_first = (int)(blockIdx.x & 3) < wta_quarters;
+    bool paths_open = true;
+    int s = 0, acquired = -1; // this wave's slot cursor; the last slot it acquired
+
+    auto try_wta = [&]() -> bool {
+        while (s < nwta) {
+            int t = 0;
+            if (threadIdx.x == 0)
+                t = __hip_atomic_load(ready + s * kCounterStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+            if (t == 0)
+                return false; // slot s not yet published
+            if (acquired < s) {
+                // pairs with the publisher's release: the task's volumes are complete
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                acquired = s;
+            }
+            const WtaTask tk = wtas[t - 1];
+            const int u = wave_fetch_add(taken + s * kCounterStride, 1);
+            if (u < tk.units) {
+                fused_wta_unit(tk, u);
+                return true;
+            }
+            s++;
+        }
+        return false;
+    };
+    auto try_path = [&]() -> bool {
+        if (!paths_open)
+            return false;
+        const int it = wave_fetch_add(head, 1);
+        if (it >= nitems) {
+            paths_open = false;
+            return false;
+        }
+        const uint32_t code = items[it];
+        batch_path_item(tasks, code);
+        const int dep = tasks[code >> 16].dep;
+        int old = 0;
+        if (threadIdx.x == 0)
+            old = __hip_atomic_fetch_add(left + dep * kCounterStride, -1, __ATOMIC_RELEASE,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        old = __builtin_amdgcn_readfirstlane(__shfl(old, 0));
+        if (old == 1) {
+            // the last item of task dep: every other item's release is ordered before this
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const int slot = wave_fetch_add(pub, 1);
+            if (threadIdx.x == 0)
+                __hip_atomic_store(ready + slot * kCounterStride, dep + 1, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return true;
+    };
+    for (;;) {
+        // one call site per kind keeps both inlined
+        bool got = false;
+        bool wta_now = wta_first;
+#pragma nounroll
+        for (int k = 0; k < 2 && !got; k++, wta_now = !wta_now)
+            got = wta_now ? try_wta() : try_path();
+        if (got)
+            continue;
+        if (s >= nwta)
+            break; // every slot published and every unit claimed
+        __builtin_amdgcn_s_sleep(8);
     }
 }
 
@@ -293,6 +447,8 @@ struct Schedule {
     int order = 0;        // 0: longest lines first; 1: pair by pair, directions in order
     int census_shape = 0; // census tile kCenShapeW[i] x kCenShapeH[i]
     int xcd_slices = 0;   // 0: one global path queue; 1: per-XCD slices
+    int fused = 0;        // 1: path and argmin in one launch (k_batch_fused; order and slices unused)
+    int wta_quarters = 1; // fused: quarters of the waves that look for argmin units first
 };
 
 } // namespace
@@ -304,10 +460,14 @@ struct BatchLane {
     DevArray<CostTask> cost[2];
     DevArray<BatchPathTask> path;
     std::vector<uint32_t> items_lpt, items_pair;
-    uint32_t *d_items = nullptr; // both orders, back to back
+    // fused launch: tasks by decreasing work, each task's items longest first
+    std::vector<uint32_t> items_task;
+    uint32_t *d_items = nullptr; // the three orders, back to back
     DevArray<WtaTask> wta;
     int pre_n[kCenShapes][3] = {}, cost_n[2] = {0, 0}, wta_n = 0; // items per launch
     int *d_counter = nullptr;
+    DevArray<int> left0; // path items per argmin task
+    int *d_sync = nullptr; // fused launch synchronisation block
     bool empty() const { return path.host.empty(); }
 };
 
@@ -398,6 +558,15 @@ int enqueue_lane(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc
             hipLaunchKernelGGL(k_batch_cost, dim3(std::min(ln.cost_n[k], pre_blocks)), dim3(256), 0, s,
                                ln.cost[k].dev, (int)ln.cost[k].host.size(), ln.cost_n[k]);
     CK(hipGetLastError());
+    if (sc.fused && !ln.path.host.empty()) {
+        const int n = (int)ln.items_lpt.size(), nwta = (int)ln.wta.host.size();
+        const int waves = std::min(n, b->cus * 4 * sc.path_waves_per_simd);
+        hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(256), 0, s, ln.d_sync, ln.left0.dev, nwta);
+        hipLaunchKernelGGL(k_batch_fused, dim3(waves), dim3(64), 0, s, ln.path.dev, ln.d_items + 2 * n, n,
+                           ln.wta.dev, nwta, ln.d_sync, sc.wta_quarters);
+        CK(hipGetLastError());
+        return 0;
+    }
     if (!ln.path.host.empty()) {
         const int n = (int)ln.items_lpt.size();
         const uint32_t *items = ln.d_items + (sc.order == 0 ? 0 : n);
@@ -465,6 +634,11 @@ int capture(sgmh_batch *b)
 // per SIMD with the default census tile and one global path queue; then, on
 // the best of those, the other census tile shape and the per-XCD scanline
 // grouping, alone and together.
+const char *order_name(const Schedule &sc)
+{
+    return sc.fused ? "task-by-task" : sc.order == 0 ? "longest-first" : "pair-major";
+}
+
 int autotune(sgmh_batch *b)
 {
     struct Cand {
@@ -492,13 +666,14 @@ int autotune(sgmh_batch *b)
                 CK(hipEventElapsedTime(&t[r], e0, e1));
             }
             std::sort(t, t + 3);
-            char buf[256];
+            char buf[320];
             snprintf(buf, sizeof buf,
                      "%s{\"lanes\":\"%s\",\"order\":\"%s\",\"path_waves_per_simd\":%d,\"census_tile\":\"%dx%d\","
-                     "\"scanline_grouping\":\"%s\",\"ms\":%.4f}",
-                     ntimed++ ? "," : "", b->sets[c.set].name.c_str(), c.sc.order == 0 ? "longest-first" : "pair-major",
+                     "\"scanline_grouping\":\"%s\",\"fused\":%s,\"argmin_first_quarters\":%d,\"ms\":%.4f}",
+                     ntimed++ ? "," : "", b->sets[c.set].name.c_str(), order_name(c.sc),
                      c.sc.path_waves_per_simd, kCenShapeW[c.sc.census_shape], kCenShapeH[c.sc.census_shape],
-                     c.sc.xcd_slices ? "per-XCD" : "global", t[1]);
+                     c.sc.xcd_slices ? "per-XCD" : "global", c.sc.fused ? "true" : "false",
+                     c.sc.fused ? c.sc.wta_quarters : 0, t[1]);
             js += buf;
             if (t[1] < best) {
                 best = t[1];
@@ -508,6 +683,9 @@ int autotune(sgmh_batch *b)
         }
         return 0;
     };
+    // The fused path + argmin launch is not a candidate: it measured slower
+    // than separate launches on every partition (its per-item device-scope
+    // release is the cost), so SGMH_BATCH_FUSED selects it explicitly.
     std::vector<Cand> stage1;
     for (int set = 0; set < (int)b->sets.size(); set++)
         for (int order = 0; order < 2; order++)
@@ -663,14 +841,18 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
             ls.lanes[li].cost[level].host.push_back(t);
         };
         auto add_wta = [&](const uint8_t *L, size_t stride, int ndirs, int npix, int D, uint8_t *disp) {
-            WtaTask t{L, stride, ndirs, npix, D, disp, wta_items[li]};
-            const int per = (64 / (D / 16)) * kWtaBatch;
-            wta_items[li] += (npix + per - 1) / per;
+            const int per = (64 / (D / 16)) * kWtaBatch, nb = (npix + per - 1) / per;
+            const int fper = (64 / (D / 16)) * (ndirs == 8 ? kFusedWtaBatch<8> : kFusedWtaBatch<4>);
+            const int fnb = (npix + fper - 1) / fper;
+            WtaTask t{L, stride, ndirs, npix, D, disp, wta_items[li], (fnb + kWtaUnit - 1) / kWtaUnit};
+            wta_items[li] += nb;
             ls.lanes[li].wta.host.push_back(t);
         };
+        // index the next add_wta call of the current lane gets (+ k)
+        auto next_wta = [&](int k) { return (int)ls.lanes[li].wta.host.size() + k; };
         auto add_path = [&](const uint64_t *cl, const uint64_t *cr, const uint8_t *cost, int W, int H, int D,
                             uint32_t P2, uint32_t inval, uint32_t out_bytes, int dx, int dy, int P1, uint8_t *L,
-                            int variant) {
+                            int variant, int dep) {
             BatchPathTask t;
             t.cl = cl;
             t.cr = cr;
@@ -690,6 +872,7 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
             t.dir.first_block = 0;
             t.dir.L = L;
             t.variant = variant;
+            t.dep = dep;
             ls.lanes[li].path.host.push_back(t);
         };
         for (int i = 0; i < n; i++) {
@@ -705,7 +888,7 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
                 for (int k = 0; k < p.paths; k++)
                     add_path(cl, cr, nullptr, W, H, D, kPlainP2, kPlainInvalid, (uint32_t)(N * D),
                              kDirsPlain[k][0], kDirsPlain[k][1], (int)kPlainP1, A + w.vol + (size_t)k * N * D,
-                             plain_variant(D));
+                             plain_variant(D), next_wta(0));
                 add_wta(A + w.vol, N * D, p.paths, (int)N, D, p.d_disp);
             } else {
                 uint8_t *hl = A + w.hl, *hr = A + w.hr, *sl = A + w.sl, *sr = A + w.sr;
@@ -728,10 +911,10 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
                 for (int k = 0; k < 8; k++) {
                     add_path(nullptr, nullptr, C1, hw, hh, D, kMsP2, kMsInvalid, (uint32_t)(Nh * D),
                              kDirsMs[k][0], kDirsMs[k][1], kDirsMs[k][2], V1 + (size_t)k * Nh * D,
-                             ms_variant(D, false));
+                             ms_variant(D, false), next_wta(0));
                     add_path(nullptr, nullptr, C2, W, H, D, kMsP2, kMsInvalid, (uint32_t)(Nh * D),
                              kDirsMs[k][0], kDirsMs[k][1], kDirsMs[k][2], V2 + (size_t)k * Nh * D,
-                             ms_variant(D, true));
+                             ms_variant(D, true), next_wta(1));
                 }
                 add_wta(V1, Nh * D, 8, (int)Nh, D, A + w.d1);
                 add_wta(V2, Nh * D, 8, (int)Nh, D, p.d_disp);
@@ -754,8 +937,14 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
             for (int k = 0; k < 2; k++)
                 ln.cost_n[k] = cost_items[L][k];
             ln.wta_n = wta_items[L];
-            // path items in two orders: longest lines first, and pair by pair
+            // path items in three orders: longest lines first; pair by pair;
+            // and (fused launch) argmin task by task, the task with the most
+            // work first and each task's longest lines first, so tasks become
+            // ready one after another and the last ones are short
             std::vector<std::pair<long, uint32_t>> all;
+            const int nwta = (int)ln.wta.host.size();
+            std::vector<long> task_work(nwta, 0);
+            ln.left0.host.assign(nwta, 0);
             for (size_t t = 0; t < ln.path.host.size(); t++) {
                 const BatchPathTask &tk = ln.path.host[t];
                 const int groups = (tk.dir.nlines + kLinesPerWave - 1) / kLinesPerWave;
@@ -763,13 +952,22 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
                     const uint32_t code = ((uint32_t)t << 16) | (uint32_t)g;
                     ln.items_pair.push_back(code);
                     all.push_back({group_work(tk, g), code});
+                    task_work[tk.dep] += all.back().first;
                 }
+                ln.left0.host[tk.dep] += groups;
             }
             REQUIRE(ln.path.host.size() < 65536, "sgmh_batch_plan: too many directions");
             std::stable_sort(all.begin(), all.end(),
                              [](const auto &x, const auto &y) { return x.first > y.first; });
             for (auto &x : all)
                 ln.items_lpt.push_back(x.second);
+            std::vector<std::pair<long, uint32_t>> by_task = all; // already longest first
+            std::stable_sort(by_task.begin(), by_task.end(), [&](const auto &x, const auto &y) {
+                const int dx = ln.path.host[x.second >> 16].dep, dy = ln.path.host[y.second >> 16].dep;
+                return task_work[dx] != task_work[dy] ? task_work[dx] > task_work[dy] : dx < dy;
+            });
+            for (auto &x : by_task)
+                ln.items_task.push_back(x.second);
         }
         ls.lanes = kept;
         b->sets.push_back(ls);
@@ -819,17 +1017,23 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
                 rc = upload(ln.path);
             if (!rc)
                 rc = upload(ln.wta);
+            if (!rc)
+                rc = upload(ln.left0);
             if (!rc && !ln.items_lpt.empty()) {
                 const size_t m = ln.items_lpt.size();
-                if (hipMalloc(&ln.d_items, 2 * m * sizeof(uint32_t)) != hipSuccess ||
+                if (hipMalloc(&ln.d_items, 3 * m * sizeof(uint32_t)) != hipSuccess ||
                     hipMemcpy(ln.d_items, ln.items_lpt.data(), m * 4, hipMemcpyHostToDevice) != hipSuccess ||
-                    hipMemcpy(ln.d_items + m, ln.items_pair.data(), m * 4, hipMemcpyHostToDevice) != hipSuccess) {
+                    hipMemcpy(ln.d_items + m, ln.items_pair.data(), m * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(ln.d_items + 2 * m, ln.items_task.data(), m * 4, hipMemcpyHostToDevice) !=
+                        hipSuccess) {
                     set_err("sgmh_batch_plan: cannot upload the path work list");
                     rc = -1;
                 }
             }
-            if (!rc && hipMalloc(&ln.d_counter, sizeof(int) * kCounterStride * kCounters) != hipSuccess) {
-                set_err("sgmh_batch_plan: cannot allocate the work counter");
+            if (!rc && (hipMalloc(&ln.d_counter, sizeof(int) * kCounterStride * kCounters) != hipSuccess ||
+                        hipMalloc(&ln.d_sync, sizeof(int) * kCounterStride * (2 + 3 * ln.wta.host.size())) !=
+                            hipSuccess)) {
+                set_err("sgmh_batch_plan: cannot allocate the work counters");
                 rc = -1;
             }
         }
@@ -848,7 +1052,8 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
         rc = autotune(b);
     } else {
         b->sched.census_shape = (flags & SGMH_BATCH_CENSUS_32X32) ? 1 : 0;
-        b->sched.xcd_slices = (flags & SGMH_BATCH_XCD_SLICES) ? 1 : 0;
+        b->sched.fused = (flags & SGMH_BATCH_FUSED) ? 1 : 0;
+        b->sched.xcd_slices = (flags & SGMH_BATCH_XCD_SLICES) && !b->sched.fused ? 1 : 0;
     }
     if (!rc && !(flags & SGMH_BATCH_NO_GRAPH))
         rc = capture(b);
@@ -907,12 +1112,13 @@ int sgmh_batch_describe(const sgmh_batch *b, char *buf, size_t len)
         buf, len,
         "{\"pairs\":%zu,\"plain\":%d,\"multiscale\":%d,\"workspace_bytes\":%zu,\"path_items\":%zu,"
         "\"path_tasks\":%zu,\"wta_tasks\":%zu,\"graph\":%s,\"schedule\":{\"lanes\":\"%s\",\"lane_count\":%zu,"
-        "\"order\":\"%s\",\"path_waves_per_simd\":%d,\"census_tile\":\"%dx%d\",\"scanline_grouping\":\"%s\"},"
-        "\"tuning\":%s}",
+        "\"order\":\"%s\",\"path_waves_per_simd\":%d,\"census_tile\":\"%dx%d\",\"scanline_grouping\":\"%s\","
+        "\"fused\":%s,\"argmin_first_quarters\":%d},\"tuning\":%s}",
         b->pairs.size(), plain, ms, b->arena_bytes, items, tasks, wtas, b->graph_ok ? "true" : "false",
-        ls.name.c_str(), ls.lanes.size(), b->sched.order == 0 ? "longest-first" : "pair-major",
+        ls.name.c_str(), ls.lanes.size(), order_name(b->sched),
         b->sched.path_waves_per_simd, kCenShapeW[b->sched.census_shape], kCenShapeH[b->sched.census_shape],
-        b->sched.xcd_slices ? "per-XCD" : "global", b->tuning.empty() ? "null" : b->tuning.c_str());
+        b->sched.xcd_slices ? "per-XCD" : "global", b->sched.fused ? "true" : "false",
+        b->sched.fused ? b->sched.wta_quarters : 0, b->tuning.empty() ? "null" : b->tuning.c_str());
     return n;
 }
 
@@ -943,6 +1149,10 @@ void sgmh_batch_destroy(sgmh_batch *b)
             (void)hipFree(ln.d_items);
         if (ln.d_counter)
             (void)hipFree(ln.d_counter);
+        if (ln.left0.dev)
+            (void)hipFree(ln.left0.dev);
+        if (ln.d_sync)
+            (void)hipFree(ln.d_sync);
         }
     if (b->arena)
         (void)hipFree(b->arena);

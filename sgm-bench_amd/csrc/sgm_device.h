@@ -956,13 +956,14 @@ __device__ __forceinline__ uint32_t group_min(uint32_t v)
 
 constexpr int kWtaBatch = 4; // pixel batches per wta_batch call: 4 x 4 directions x 16 B in flight per lane
 
-// One batch of U*PPW pixels starting at b0 (every lane of the wave calls it).
-template <int LPP, int NDIRS>
+// One batch of U*PPW pixels starting at b0 (every lane of the wave calls it
+// with its lane index).
+template <int LPP, int NDIRS, int U = kWtaBatch>
 __device__ __forceinline__ void wta_batch(const uint8_t *__restrict__ L, size_t dir_stride, int npix,
-                                          uint8_t *__restrict__ disp, int b0)
+                                          uint8_t *__restrict__ disp, int b0, int lane)
 {
-    constexpr int D = 16 * LPP, PPW = 64 / LPP, U = kWtaBatch;
-    const int lane = threadIdx.x & 63, sub = lane % LPP, slot = lane / LPP;
+    constexpr int D = 16 * LPP, PPW = 64 / LPP;
+    const int sub = lane % LPP, slot = lane / LPP;
     uint4 v[U][NDIRS];
     int pix[U];
 #pragma unroll

@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void k_wta(const uint8_t *__restrict__ L, size
     const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     const int nwaves = (int)gridDim.x * 4;
     for (int b0 = wave * PPW * kWtaBatch; b0 < npix; b0 += nwaves * PPW * kWtaBatch)
-        wta_batch<LPP, NDIRS>(L, dir_stride, npix, disp, b0);
+        wta_batch<LPP, NDIRS>(L, dir_stride, npix, disp, b0, threadIdx.x & 63);
 }
 
 // windowed SGM: every direction of the frame in one launch, one line group

@@ -43,7 +43,9 @@ SMALL = [
 @pytest.mark.parametrize("flags", [sgm_amd.BATCH_AUTOTUNE, sgm_amd.BATCH_NO_GRAPH, 0,
                                    # the schedule choices the autotuner may make, pinned
                                    sgm_amd.BATCH_CENSUS_32X32, sgm_amd.BATCH_XCD_SLICES,
-                                   sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES | sgm_amd.BATCH_NO_GRAPH])
+                                   sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES | sgm_amd.BATCH_NO_GRAPH,
+                                   sgm_amd.BATCH_FUSED, sgm_amd.BATCH_FUSED | sgm_amd.BATCH_NO_GRAPH,
+                                   sgm_amd.BATCH_FUSED | sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES])
 def test_mixed_batch_matches_oracle(ctx, flags):
     data = []
     for sp in SMALL:
@@ -70,19 +72,26 @@ def test_mixed_batch_matches_oracle(ctx, flags):
         assert np.array_equal(d_out.cpu().numpy(), w), f"loop: {sp}"
     info = b.describe()
     assert info["pairs"] == len(SMALL)
+    sched = info["schedule"]
     if not flags & sgm_amd.BATCH_AUTOTUNE:
-        sched = info["schedule"]
+        fused = bool(flags & sgm_amd.BATCH_FUSED)
         assert sched["census_tile"] == ("32x32" if flags & sgm_amd.BATCH_CENSUS_32X32 else "64x16")
-        assert sched["scanline_grouping"] == ("per-XCD" if flags & sgm_amd.BATCH_XCD_SLICES else "global")
+        # the fused launch has one global queue whatever the flags ask
+        assert sched["scanline_grouping"] == ("per-XCD" if flags & sgm_amd.BATCH_XCD_SLICES and not fused
+                                              else "global")
+        assert sched["fused"] == fused
     else:
-        # the autotuner timed both census tile shapes and both groupings
+        # the autotuner timed both census tile shapes and both groupings, all
+        # with separate path and argmin launches
         tried = {(t["census_tile"], t["scanline_grouping"]) for t in info["tuning"]}
         assert tried == {(c, g) for c in ("64x16", "32x32") for g in ("global", "per-XCD")}
+        assert not any(t["fused"] for t in info["tuning"]) and not sched["fused"]
     b.close()
 
 
 @pytest.mark.parametrize("flags", [sgm_amd.BATCH_AUTOTUNE,
-                                   sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES])
+                                   sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES,
+                                   sgm_amd.BATCH_FUSED])
 def test_full_batch_goldens(ctx, goldens, flags):
     data, specs = [], sgm_amd.batch_spec()
     for sp in specs:
