@@ -218,6 +218,8 @@ typedef struct sgmh_batch sgmh_batch;
 #define SGMH_BATCH_XCD_SLICES 8   /* path line groups taken from per-XCD slices of the queue */
 #define SGMH_BATCH_FUSED 16       /* path aggregation and sum + argmin in one launch, each pair's
                                      argmin as soon as its directions are done (ignores XCD_SLICES) */
+#define SGMH_BATCH_PIPELINED 32   /* groups of pairs pipelined over three streams: one group's
+                                     argmin and the next's cost volumes beside the path launches */
 
 /* Plan a batch: allocate every pair's workspace, build the work lists of the
  * persistent per-phase kernels (pre-filter / census, cost, path aggregation,

@@ -203,6 +203,8 @@ __device__ __forceinline__ void batch_path_item(const BatchPathTask *__restrict_
     case kVarMsDec64: path_group<4, false, true, true>(tk, tk.dir, grp); break;
     default: path_group<8, false, true, true>(tk, tk.dir, grp); break;
     }
+    // a row sets its wave's issue priority for the row alone
+    __builtin_amdgcn_s_setprio(0);
 }
 
 // Scanline groupings of the path work list the autotuner picks between:
@@ -474,12 +476,20 @@ struct BatchLane {
 // A partition of the pairs into lanes.  Lanes share no data, so their chains
 // run concurrently and one lane's memory-bound phases (census, cost volumes,
 // argmin) overlap another's arithmetic-bound path aggregation.
+//
+// A pipelined set orders its lanes as groups instead: one stream runs every
+// group's path launch in turn, a second every group's census and cost
+// volumes ahead of it, and a third every group's argmin behind it, so group
+// k's argmin and group k+1's cost volumes overlap path aggregation, and
+// each hand-over is a launch boundary.
 struct LaneSet {
     std::string name;
     std::vector<BatchLane> lanes;
+    bool pipelined = false;
 };
 
-constexpr int kMaxLanes = 4;
+constexpr int kMaxLanes = 4;  // concurrent lane chains (one stream each)
+constexpr int kMaxGroups = 8; // lanes of a pipelined set
 
 struct sgmh_batch {
     sgmh_ctx *ctx = nullptr;
@@ -497,6 +507,7 @@ struct sgmh_batch {
     hipStream_t cap = nullptr;
     hipStream_t side[kMaxLanes - 1] = {};
     hipEvent_t fork = nullptr, join[kMaxLanes - 1] = {};
+    hipEvent_t ev_pre[kMaxGroups] = {}, ev_path[kMaxGroups] = {}; // pipelined hand-overs
 };
 
 namespace {
@@ -539,7 +550,8 @@ long group_work(const BatchPathTask &t, int grp)
     return best * t.D;
 }
 
-int enqueue_lane(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc)
+// census / half / Sobel and the multiscale cost volumes of one lane
+int enqueue_precost(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc)
 {
     const int pre_blocks = b->cus * 8;
     const int sh = sc.census_shape;
@@ -558,6 +570,38 @@ int enqueue_lane(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc
             hipLaunchKernelGGL(k_batch_cost, dim3(std::min(ln.cost_n[k], pre_blocks)), dim3(256), 0, s,
                                ln.cost[k].dev, (int)ln.cost[k].host.size(), ln.cost_n[k]);
     CK(hipGetLastError());
+    return 0;
+}
+
+int enqueue_path(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc)
+{
+    if (ln.path.host.empty())
+        return 0;
+    const int n = (int)ln.items_lpt.size();
+    const uint32_t *items = ln.d_items + (sc.order == 0 ? 0 : n);
+    const int waves = std::min(n, b->cus * 4 * sc.path_waves_per_simd);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, ln.d_counter);
+    if (sc.xcd_slices)
+        hipLaunchKernelGGL(k_batch_path_xcd, dim3(waves), dim3(64), 0, s, ln.path.dev, items, n, ln.d_counter);
+    else
+        hipLaunchKernelGGL(k_batch_path, dim3(waves), dim3(64), 0, s, ln.path.dev, items, n, ln.d_counter);
+    CK(hipGetLastError());
+    return 0;
+}
+
+int enqueue_wta(sgmh_batch *b, BatchLane &ln, hipStream_t s)
+{
+    if (ln.wta_n)
+        hipLaunchKernelGGL(k_batch_wta, dim3(std::min((ln.wta_n + 3) / 4, b->cus * 8)), dim3(256), 0, s,
+                           ln.wta.dev, (int)ln.wta.host.size(), ln.wta_n);
+    CK(hipGetLastError());
+    return 0;
+}
+
+int enqueue_lane(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc)
+{
+    if (enqueue_precost(b, ln, s, sc))
+        return -1;
     if (sc.fused && !ln.path.host.empty()) {
         const int n = (int)ln.items_lpt.size(), nwta = (int)ln.wta.host.size();
         const int waves = std::min(n, b->cus * 4 * sc.path_waves_per_simd);
@@ -567,28 +611,49 @@ int enqueue_lane(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc
         CK(hipGetLastError());
         return 0;
     }
-    if (!ln.path.host.empty()) {
-        const int n = (int)ln.items_lpt.size();
-        const uint32_t *items = ln.d_items + (sc.order == 0 ? 0 : n);
-        const int waves = std::min(n, b->cus * 4 * sc.path_waves_per_simd);
-        hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, ln.d_counter);
-        if (sc.xcd_slices)
-            hipLaunchKernelGGL(k_batch_path_xcd, dim3(waves), dim3(64), 0, s, ln.path.dev, items, n, ln.d_counter);
-        else
-            hipLaunchKernelGGL(k_batch_path, dim3(waves), dim3(64), 0, s, ln.path.dev, items, n, ln.d_counter);
+    if (enqueue_path(b, ln, s, sc))
+        return -1;
+    return enqueue_wta(b, ln, s);
+}
+
+// A pipelined set: path launches in group order on `s`; census and cost
+// volumes on side[0] ahead of them, argmin on side[1] behind them, ordered by
+// events; both side streams join `s` at the end.
+int enqueue_pipelined(sgmh_batch *b, hipStream_t s, std::vector<BatchLane> &groups, const Schedule &sc)
+{
+    hipStream_t pre = b->side[0], arg = b->side[1];
+    const int G = (int)groups.size();
+    CK(hipEventRecord(b->fork, s));
+    CK(hipStreamWaitEvent(pre, b->fork, 0));
+    CK(hipStreamWaitEvent(arg, b->fork, 0));
+    for (int k = 0; k < G; k++) {
+        if (enqueue_precost(b, groups[k], pre, sc))
+            return -1;
+        CK(hipEventRecord(b->ev_pre[k], pre));
     }
-    if (ln.wta_n)
-        hipLaunchKernelGGL(k_batch_wta, dim3(std::min((ln.wta_n + 3) / 4, b->cus * 8)), dim3(256), 0, s,
-                           ln.wta.dev, (int)ln.wta.host.size(), ln.wta_n);
-    CK(hipGetLastError());
+    for (int k = 0; k < G; k++) {
+        CK(hipStreamWaitEvent(s, b->ev_pre[k], 0));
+        if (enqueue_path(b, groups[k], s, sc))
+            return -1;
+        CK(hipEventRecord(b->ev_path[k], s));
+        CK(hipStreamWaitEvent(arg, b->ev_path[k], 0));
+        if (enqueue_wta(b, groups[k], arg))
+            return -1;
+    }
+    CK(hipEventRecord(b->join[0], pre));
+    CK(hipEventRecord(b->join[1], arg));
+    CK(hipStreamWaitEvent(s, b->join[0], 0));
+    CK(hipStreamWaitEvent(s, b->join[1], 0));
     return 0;
 }
 
 // Every lane of the chosen set: lane 0 on `s`, the others on side streams
-// forked from it and joined back.
+// forked from it and joined back (or the pipelined order above).
 int enqueue_phases(sgmh_batch *b, hipStream_t s, int set, const Schedule &sc)
 {
     std::vector<BatchLane> &lanes = b->sets[set].lanes;
+    if (b->sets[set].pipelined)
+        return enqueue_pipelined(b, s, lanes, sc);
     const int K = (int)lanes.size();
     if (K == 1)
         return enqueue_lane(b, lanes[0], s, sc);
@@ -724,6 +789,8 @@ extern "C" {
 int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgmh_batch **out)
 {
     REQUIRE(ctx && pairs && out && n > 0, "sgmh_batch_plan: bad arguments");
+    REQUIRE(!((flags & SGMH_BATCH_FUSED) && (flags & SGMH_BATCH_PIPELINED)),
+            "sgmh_batch_plan: SGMH_BATCH_FUSED and SGMH_BATCH_PIPELINED exclude each other");
     *out = nullptr;
     for (int i = 0; i < n; i++) {
         const sgmh_pair &p = pairs[i];
@@ -809,7 +876,7 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
         LaneSet ls;
         ls.name = name;
         ls.lanes.resize(nl);
-        int pre_items[kMaxLanes][kCenShapes][3] = {}, cost_items[kMaxLanes][2] = {}, wta_items[kMaxLanes] = {};
+        int pre_items[kMaxGroups][kCenShapes][3] = {}, cost_items[kMaxGroups][2] = {}, wta_items[kMaxGroups] = {};
         int li = 0;
         auto add_pre = [&](int level, int kind, const uint8_t *src, void *dst, int W, int H, int ostride) {
             for (int sh = 0; sh < kCenShapes; sh++) {
@@ -922,7 +989,7 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
         }
         // drop empty lanes
         std::vector<BatchLane> kept;
-        int kept_idx[kMaxLanes];
+        int kept_idx[kMaxGroups];
         for (int L = 0; L < nl; L++)
             if (!ls.lanes[L].path.host.empty()) {
                 kept_idx[kept.size()] = L;
@@ -1006,6 +1073,58 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
         snprintf(name, sizeof name, "balanced-%d", K);
         rc = build(lane_of, K, name);
     }
+    // Pipelined sets: the pairs in decreasing path work (line steps x D:
+    // paths * N * D plain; 6 * N * D multiscale, a quarter-size stage and a
+    // stage that aggregates half the full-size lines, 8 directions each), cut
+    // into G contiguous groups of about equal work (a pair joins the group its
+    // work's midpoint falls in).  Multiscale pairs in the first group trade
+    // places with the next plain pairs, so the first path launch waits only
+    // for census.
+    const bool pipe_flag = (flags & SGMH_BATCH_PIPELINED) != 0;
+    int first_pipelined = -1;
+    if ((tune || pipe_flag) && !rc) {
+        std::vector<double> pwork(n);
+        double total = 0.0;
+        for (int i = 0; i < n; i++) {
+            const sgmh_pair &p = pairs[i];
+            const double nd = (double)p.W * p.H * p.D;
+            pwork[i] = p.mode == SGMH_PLAIN ? nd * p.paths : nd * 6.0;
+            total += pwork[i];
+        }
+        std::vector<int> order(n);
+        for (int i = 0; i < n; i++)
+            order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pwork[x] > pwork[y]; });
+        for (int G : {3, 4, 6}) {
+            if (G > n || rc)
+                break;
+            std::vector<int> lane_of(n);
+            double cum = 0.0;
+            for (int i : order) {
+                lane_of[i] = std::min(G - 1, (int)(G * (cum + 0.5 * pwork[i]) / total));
+                cum += pwork[i];
+            }
+            size_t next_plain = 0;
+            for (int i : order) {
+                if (lane_of[i] != 0 || pairs[i].mode == SGMH_PLAIN)
+                    continue;
+                while (next_plain < order.size() &&
+                       (lane_of[order[next_plain]] == 0 || pairs[order[next_plain]].mode != SGMH_PLAIN))
+                    next_plain++;
+                if (next_plain == order.size())
+                    break;
+                std::swap(lane_of[i], lane_of[order[next_plain]]);
+            }
+            char name[32];
+            snprintf(name, sizeof name, "pipelined-%d", G);
+            rc = build(lane_of, G, name);
+            if (!rc) {
+                b->sets.back().pipelined = true;
+                if (first_pipelined < 0)
+                    first_pipelined = (int)b->sets.size() - 1;
+            }
+        }
+    }
     for (LaneSet &ls : b->sets)
         for (BatchLane &ln : ls.lanes) {
             for (int sh = 0; sh < kCenShapes; sh++)
@@ -1048,12 +1167,20 @@ int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgm
             set_err("sgmh_batch_plan: cannot create streams / events");
             rc = -1;
         }
+    for (int k = 0; k < kMaxGroups && !rc; k++)
+        if (hipEventCreateWithFlags(&b->ev_pre[k], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&b->ev_path[k], hipEventDisableTiming) != hipSuccess) {
+            set_err("sgmh_batch_plan: cannot create events");
+            rc = -1;
+        }
     if (!rc && (flags & SGMH_BATCH_AUTOTUNE)) {
         rc = autotune(b);
     } else {
         b->sched.census_shape = (flags & SGMH_BATCH_CENSUS_32X32) ? 1 : 0;
         b->sched.fused = (flags & SGMH_BATCH_FUSED) ? 1 : 0;
         b->sched.xcd_slices = (flags & SGMH_BATCH_XCD_SLICES) && !b->sched.fused ? 1 : 0;
+        if (pipe_flag && first_pipelined >= 0)
+            b->set = first_pipelined;
     }
     if (!rc && !(flags & SGMH_BATCH_NO_GRAPH))
         rc = capture(b);
@@ -1112,10 +1239,10 @@ int sgmh_batch_describe(const sgmh_batch *b, char *buf, size_t len)
         buf, len,
         "{\"pairs\":%zu,\"plain\":%d,\"multiscale\":%d,\"workspace_bytes\":%zu,\"path_items\":%zu,"
         "\"path_tasks\":%zu,\"wta_tasks\":%zu,\"graph\":%s,\"schedule\":{\"lanes\":\"%s\",\"lane_count\":%zu,"
-        "\"order\":\"%s\",\"path_waves_per_simd\":%d,\"census_tile\":\"%dx%d\",\"scanline_grouping\":\"%s\","
+        "\"pipelined\":%s,\"order\":\"%s\",\"path_waves_per_simd\":%d,\"census_tile\":\"%dx%d\",\"scanline_grouping\":\"%s\","
         "\"fused\":%s,\"argmin_first_quarters\":%d},\"tuning\":%s}",
         b->pairs.size(), plain, ms, b->arena_bytes, items, tasks, wtas, b->graph_ok ? "true" : "false",
-        ls.name.c_str(), ls.lanes.size(), order_name(b->sched),
+        ls.name.c_str(), ls.lanes.size(), ls.pipelined ? "true" : "false", order_name(b->sched),
         b->sched.path_waves_per_simd, kCenShapeW[b->sched.census_shape], kCenShapeH[b->sched.census_shape],
         b->sched.xcd_slices ? "per-XCD" : "global", b->sched.fused ? "true" : "false",
         b->sched.fused ? b->sched.wta_quarters : 0, b->tuning.empty() ? "null" : b->tuning.c_str());
@@ -1158,6 +1285,12 @@ void sgmh_batch_destroy(sgmh_batch *b)
         (void)hipFree(b->arena);
     if (b->cap)
         (void)hipStreamDestroy(b->cap);
+    for (int k = 0; k < kMaxGroups; k++) {
+        if (b->ev_pre[k])
+            (void)hipEventDestroy(b->ev_pre[k]);
+        if (b->ev_path[k])
+            (void)hipEventDestroy(b->ev_path[k]);
+    }
     for (int k = 0; k < kMaxLanes - 1; k++) {
         if (b->side[k])
             (void)hipStreamDestroy(b->side[k]);

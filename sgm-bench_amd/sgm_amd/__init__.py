@@ -86,6 +86,7 @@ class Pair(ctypes.Structure):
 
 PLAIN, MULTISCALE = 0, 1
 BATCH_AUTOTUNE, BATCH_NO_GRAPH, BATCH_CENSUS_32X32, BATCH_XCD_SLICES, BATCH_FUSED = 1, 2, 4, 8, 16
+BATCH_PIPELINED = 32
 LAYOUT_AUTO, LAYOUT_LINES4, LAYOUT_WAVE, LAYOUT_PLACED = 0, 1, 2, 3  # SGMH_LAYOUT_* (include/sgm_hip.h)
 
 

@@ -45,7 +45,9 @@ SMALL = [
                                    sgm_amd.BATCH_CENSUS_32X32, sgm_amd.BATCH_XCD_SLICES,
                                    sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES | sgm_amd.BATCH_NO_GRAPH,
                                    sgm_amd.BATCH_FUSED, sgm_amd.BATCH_FUSED | sgm_amd.BATCH_NO_GRAPH,
-                                   sgm_amd.BATCH_FUSED | sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES])
+                                   sgm_amd.BATCH_FUSED | sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES,
+                                   sgm_amd.BATCH_PIPELINED, sgm_amd.BATCH_PIPELINED | sgm_amd.BATCH_NO_GRAPH,
+                                   sgm_amd.BATCH_PIPELINED | sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES])
 def test_mixed_batch_matches_oracle(ctx, flags):
     data = []
     for sp in SMALL:
@@ -80,18 +82,20 @@ def test_mixed_batch_matches_oracle(ctx, flags):
         assert sched["scanline_grouping"] == ("per-XCD" if flags & sgm_amd.BATCH_XCD_SLICES and not fused
                                               else "global")
         assert sched["fused"] == fused
+        assert sched["pipelined"] == bool(flags & sgm_amd.BATCH_PIPELINED)
     else:
         # the autotuner timed both census tile shapes and both groupings, all
         # with separate path and argmin launches
         tried = {(t["census_tile"], t["scanline_grouping"]) for t in info["tuning"]}
         assert tried == {(c, g) for c in ("64x16", "32x32") for g in ("global", "per-XCD")}
         assert not any(t["fused"] for t in info["tuning"]) and not sched["fused"]
+        assert {t["lanes"] for t in info["tuning"]} >= {"pipelined-3", "pipelined-4", "pipelined-6"}
     b.close()
 
 
 @pytest.mark.parametrize("flags", [sgm_amd.BATCH_AUTOTUNE,
                                    sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES,
-                                   sgm_amd.BATCH_FUSED])
+                                   sgm_amd.BATCH_FUSED, sgm_amd.BATCH_PIPELINED])
 def test_full_batch_goldens(ctx, goldens, flags):
     data, specs = [], sgm_amd.batch_spec()
     for sp in specs:
@@ -109,6 +113,11 @@ def test_full_batch_goldens(ctx, goldens, flags):
 
 
 def test_batch_rejects_unsupported(ctx):
+    ok = torch.zeros((64, 64), dtype=torch.uint8, device="cuda")
+    pair = dict(d_left=ok.data_ptr(), d_right=ok.data_ptr(), d_disp=ok.data_ptr(), W=64, H=64, D=64,
+                mode=sgm_amd.PLAIN, paths=4)
+    with pytest.raises(sgm_amd.SgmError, match="exclude each other"):
+        ctx.batch([pair], sgm_amd.BATCH_FUSED | sgm_amd.BATCH_PIPELINED)
     L = torch.zeros((64, 64), dtype=torch.uint8, device="cuda")
     bad = dict(d_left=L.data_ptr(), d_right=L.data_ptr(), d_disp=L.data_ptr(), W=64, H=64, D=256,
                mode=sgm_amd.PLAIN, paths=4)
