@@ -75,6 +75,13 @@ def test_host_programs_usage(prog):
     assert r.returncode == 1 and "usage" in r.stderr
 
 
+def test_batch_program_usage():
+    """sgm_batch_hip runs its built-in batch without arguments, so an unknown
+    argument is what prints its usage (exit 1, before any GPU work)."""
+    r = _run(os.path.join(BIN, "sgm_batch_hip"), "--bogus")
+    assert r.returncode == 1 and "usage" in r.stderr
+
+
 def test_harness_rejects_bad_input_before_any_gpu_work(tmp_path):
     """Unreadable files, a corrupt PGM header and mismatched sizes exit 1 on the
     host (common/util.c:23-33, common/harness.c:142-145)."""

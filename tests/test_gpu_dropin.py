@@ -23,6 +23,9 @@ CAL = os.path.join(ROOT, "sgm-bench_amd", "host", "roofline_mi355x.cal")
 # scripts/record_ms.sh:25-26, the reference's parser of the multiscale result line
 MS_LINE = re.compile(r"^(\S+)\s+in (\d+)x(\d+) out (\d+)x(\d+) D=(\d+) paths=(\d+)(?: th=(\d+))?\s+"
                      r"median ([\d.]+) ms\s+fps ([\d.]+)\s+MDE/s\(work\) (\d+)\s+hash (\w+)(.*)$")
+# the batch binary's result line
+BATCH_LINE = re.compile(r"^batch_hip\s+pairs (\d+) \((\d+) plain, (\d+) multiscale\)\s+median ([\d.]+) ms\s+"
+                        r"frames/s ([\d.]+)\s+loop ([\d.]+) ms\s+speedup ([\d.]+)\s+MDE/s (\d+)(.*)$")
 # common/harness.c:217-229, the JSON record's keys in order
 RECORD_KEYS = ["impl", "board", "W", "H", "D", "paths", "census", "P1", "P2", "threads", "threads_requested",
                "cpu_mask", "warm", "timed", "median_ms", "p95_ms", "min_ms", "rmsd_ms", "spread_hi", "spread_lo",
@@ -140,3 +143,37 @@ def test_ms_binary_on_reference_goldens(tmp_path, H, fnv):
     sgm_amd.pgm_write(gp, bad)
     r = run(os.path.join(BIN, "sgm_ms_hip"), lp, rp, "-g", gp, "-n", "1")
     assert r.returncode == 2, r.stdout + r.stderr
+
+
+def test_batch_binary_goldens(tmp_path, goldens):
+    """sgm_batch_hip builds the 24-pair batch from its seeds, autotunes it and
+    hashes every pair: GOLDEN OK and exit 0 against the fixtures, and exit 2
+    once one pair's golden is changed (a pinned schedule, no autotuning)."""
+    gold = {g["name"]: g["fnv"] for g in goldens["batch"]}
+    gp, js = str(tmp_path / "goldens.txt"), str(tmp_path / "rec.jsonl")
+
+    def write(g):
+        with open(gp, "w") as f:
+            for k, v in g.items():
+                f.write(f"{k} {v}\n")
+
+    exe = os.path.join(BIN, "sgm_batch_hip")
+    write(gold)
+    r = run(exe, "-g", gp, "-n", "5", "-j", js, "-v")
+    assert r.returncode == 0, r.stdout + r.stderr
+    line = [x for x in r.stdout.splitlines() if x.startswith("batch_hip")][0]
+    m = BATCH_LINE.search(line)
+    assert m, line
+    n, nplain, nms, med, fps, loop, speedup, mde, tail = m.groups()
+    assert (int(n), int(nplain), int(nms)) == (24, 20, 4) and "GOLDEN OK" in tail
+    assert abs(float(loop) / float(med) - float(speedup)) < 2e-3
+    hashes = dict(x.split()[0::5] for x in r.stdout.splitlines() if " hash " in x and not x.startswith("batch_hip"))
+    assert hashes == gold
+    rec = json.loads(open(js).read().splitlines()[-1])
+    assert rec["impl"] == "batch_hip" and rec["golden_match"] == 1 and rec["plan"]["pairs"] == 24
+    assert rec["plan"]["tuning"], "the default plan is autotuned"
+    first = next(iter(gold))
+    write({**gold, first: "%016x" % (int(gold[first], 16) ^ 1)})
+    r = run(exe, "-g", gp, "-n", "1", "-f", "0")
+    assert r.returncode == 2 and "GOLDEN FAIL" in r.stdout, r.stdout + r.stderr
+    assert first in r.stderr
