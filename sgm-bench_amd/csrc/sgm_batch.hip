@@ -383,8 +383,10 @@ __global__ __launch_bounds__(64) void k_batch_fused(const BatchPathTask *__restr
             if (t == 0)
                 return false; // slot s not yet published
             if (acquired < s) {
-                // pairs with the publisher's release: the task's volumes are complete
+                // pairs with the publisher's release: the task's volumes are
+                // complete (the barrier extends lane 0's acquire to the wave)
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                __syncthreads();
                 acquired = s;
             }
             const WtaTask tk = wtas[t - 1];
@@ -408,6 +410,7 @@ __global__ __launch_bounds__(64) void k_batch_fused(const BatchPathTask *__restr
         const uint32_t code = items[it];
         batch_path_item(tasks, code);
         const int dep = tasks[code >> 16].dep;
+        __syncthreads(); // every lane's volume stores happen before lane 0's release
         int old = 0;
         if (threadIdx.x == 0)
             old = __hip_atomic_fetch_add(left + dep * kCounterStride, -1, __ATOMIC_RELEASE,
