@@ -93,6 +93,39 @@ def test_mixed_batch_matches_oracle(ctx, flags):
     b.close()
 
 
+@pytest.mark.parametrize("seed,flags", [(11, 0), (12, sgm_amd.BATCH_FUSED), (13, sgm_amd.BATCH_PIPELINED),
+                                        (14, sgm_amd.BATCH_AUTOTUNE)])
+def test_random_batches(ctx, seed, flags):
+    """Random compositions: 5-9 pairs of random (ragged) size, D, configuration
+    and path count, through one schedule each, every pair against its oracle
+    on two consecutive runs."""
+    rng = np.random.default_rng(seed)
+    data = []
+    for i in range(int(rng.integers(5, 10))):
+        mode = sgm_amd.PLAIN if rng.random() < 0.6 else sgm_amd.MULTISCALE
+        D = int(rng.choice([32, 64, 128]))
+        if mode == sgm_amd.PLAIN:
+            W, H, paths = int(rng.integers(17, 180)), int(rng.integers(16, 120)), int(rng.choice([4, 8]))
+        else:
+            W, H, paths = 2 * int(rng.integers(16, 90)), 2 * int(rng.integers(16, 60)), 8
+        sp = dict(W=W, H=H, D=D, mode=mode, paths=paths)
+        L, R, _ = sgm_amd.scene_planes(W, H, min(D, W) - 1, seed * 100 + i, planes=(2, 6))
+        data.append((sp, L, R))
+    pairs, keep = upload(data)
+    b = ctx.batch(pairs, flags)
+    s = torch.cuda.current_stream().cuda_stream
+    wants = [want(sp, L, R) for sp, L, R in data]
+    for rep in range(2):
+        for t in keep[2::3]:
+            t.fill_(0xEE)
+        b.run(s)
+        torch.cuda.synchronize()
+        for (sp, _, _), w, d_out in zip(data, wants, keep[2::3]):
+            got = d_out.cpu().numpy()
+            assert np.array_equal(got, w), f"run {rep}: {sp}: {(got != w).sum()} pixels differ"
+    b.close()
+
+
 @pytest.mark.parametrize("flags", [sgm_amd.BATCH_AUTOTUNE,
                                    sgm_amd.BATCH_CENSUS_32X32 | sgm_amd.BATCH_XCD_SLICES,
                                    sgm_amd.BATCH_FUSED, sgm_amd.BATCH_PIPELINED])
