@@ -212,22 +212,23 @@ typedef struct sgmh_batch sgmh_batch;
 
 #define SGMH_BATCH_AUTOTUNE 1 /* time candidate schedules on this batch, keep the best */
 #define SGMH_BATCH_NO_GRAPH 2 /* launch the phase kernels directly, no hipGraph */
-/* Without SGMH_BATCH_AUTOTUNE, pin the two schedule choices the autotuner
- * otherwise makes: */
+/* Without SGMH_BATCH_AUTOTUNE, these pin the schedule (the first two are
+ * choices the autotuner also makes; FUSED and PIPELINED exclude each other): */
 #define SGMH_BATCH_CENSUS_32X32 4 /* census in 32x32 tiles instead of 64x16 */
 #define SGMH_BATCH_XCD_SLICES 8   /* path line groups taken from per-XCD slices of the queue */
 #define SGMH_BATCH_FUSED 16       /* path aggregation and sum + argmin in one launch, each pair's
                                      argmin as soon as its directions are done (ignores XCD_SLICES) */
-#define SGMH_BATCH_PIPELINED 32   /* groups of pairs pipelined over three streams: one group's
-                                     argmin and the next's cost volumes beside the path launches */
+#define SGMH_BATCH_PIPELINED 32   /* three groups of pairs pipelined over three streams: one group's
+                                     argmin and the next's cost volumes beside the path launches
+                                     (the autotuner also times 3, 4 and 6 groups) */
 
 /* Plan a batch: allocate every pair's workspace, build the work lists of the
  * persistent per-phase kernels (pre-filter / census, cost, path aggregation,
- * sum + argmin), optionally autotune the schedule (lane partition, path item
- * order, path waves per SIMD, census tile shape, scanline grouping, separate
- * or fused path and argmin launches), and
- * capture the phases as one hipGraph.  The device buffers named in `pairs` are bound into the plan:
- * later runs read whatever they then hold. */
+ * sum + argmin), optionally autotune the schedule (lane partition or pipelined
+ * groups, path item order, path waves per SIMD, census tile shape, scanline
+ * grouping), and capture the phases as one hipGraph.  The device buffers
+ * named in `pairs` are bound into the plan: later runs read whatever they
+ * then hold. */
 int sgmh_batch_plan(sgmh_ctx *ctx, const sgmh_pair *pairs, int n, int flags, sgmh_batch **out);
 /* Run the planned batch on `stream` (NULL: the HIP default stream), asynchronous. */
 int sgmh_batch_run(sgmh_batch *b, void *stream);
@@ -235,7 +236,9 @@ int sgmh_batch_run(sgmh_batch *b, void *stream);
  * call after another, on `stream`. */
 int sgmh_batch_run_loop(sgmh_batch *b, void *stream);
 /* JSON description of the plan: pairs, items per phase, chosen schedule and
- * the autotuner's timings.  Returns the length written (truncated to len-1). */
+ * the autotuner's timings, written into buf (at most len-1 characters and a
+ * terminating NUL).  Returns the description's full length, as snprintf
+ * does: a value >= len means it was truncated; -1 on a NULL argument. */
 int sgmh_batch_describe(const sgmh_batch *b, char *buf, size_t len);
 void sgmh_batch_destroy(sgmh_batch *b);
 

@@ -456,7 +456,12 @@ class Batch:
     def describe(self) -> dict:
         import json
         buf = ctypes.create_string_buffer(65536)
-        self._lib.sgmh_batch_describe(self._h, buf, len(buf))
+        n = self._lib.sgmh_batch_describe(self._h, buf, len(buf))
+        if n >= len(buf):  # truncated: the return value is the full length
+            buf = ctypes.create_string_buffer(n + 1)
+            n = self._lib.sgmh_batch_describe(self._h, buf, len(buf))
+        if n < 0:
+            raise SgmError("sgmh_batch_describe failed")
         return json.loads(buf.value.decode())
 
     def close(self) -> None:
