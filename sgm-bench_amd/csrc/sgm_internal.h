@@ -47,6 +47,8 @@ struct sgmh_ctx {
     size_t ev_used = 0;
     std::vector<std::pair<int, std::pair<int, int>>> marks; // phase, ev begin, ev end
     hipStream_t last_stream = nullptr;
+    int chain_ev = -1;            // end event of the last phase, which the next one on
+    hipStream_t chain_s = nullptr; // the same stream starts from
     // workspaces
     DevBuf img_l, img_r, disp, disp1;          // host-buffer entry points
     DevBuf cl, cr;                             // census codes
@@ -80,7 +82,10 @@ inline void free_buf(DevBuf &b)
 }
 
 // Phase timing: events are recorded only when timing is on (and never
-// without a context).
+// without a context).  A phase that follows another on the same stream starts
+// at that one's end event: every entry point's phases are contiguous (no
+// untimed work between them on one stream), and each event record costs the
+// host-buffer frame about 7 us.
 struct PhaseScope {
     sgmh_ctx *ctx;
     hipStream_t s;
@@ -88,13 +93,16 @@ struct PhaseScope {
     PhaseScope(sgmh_ctx *c, hipStream_t st, int ph) : ctx(c), s(st), phase(ph)
     {
         if (ctx && ctx->timing) // no context: nothing is timed
-            ev0 = record();
+            ev0 = ctx->chain_ev >= 0 && ctx->chain_s == s ? ctx->chain_ev : record();
     }
     int record()
     {
         if (ctx->ev_used == ctx->events.size()) {
+            // timing only: no system-scope fence (a cache write-back and
+            // invalidate) at each record; the entry points that hand results
+            // to the host synchronise their stream, which fences
             hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess)
+            if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess)
                 return -1;
             ctx->events.push_back(e);
         }
@@ -107,8 +115,11 @@ struct PhaseScope {
     {
         if (ctx && ctx->timing && ev0 >= 0) {
             const int ev1 = record();
-            if (ev1 >= 0)
+            if (ev1 >= 0) {
                 ctx->marks.push_back({phase, {ev0, ev1}});
+                ctx->chain_ev = ev1;
+                ctx->chain_s = s;
+            }
         }
     }
 };
@@ -123,6 +134,8 @@ inline void begin_call(sgmh_ctx *ctx, hipStream_t s)
     ctx->ev_used = 0;
     ctx->marks.clear();
     ctx->last_stream = s;
+    ctx->chain_ev = -1;
+    ctx->chain_s = nullptr;
 }
 
 inline int check_dims(int W, int H, int D)
