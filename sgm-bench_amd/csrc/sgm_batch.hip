@@ -308,8 +308,8 @@ __global__ __launch_bounds__(256) void k_batch_wta(const WtaTask *__restrict__ t
 // that takes the count to zero publishes the task in the next ready slot.
 // Waves then take argmin units from ready slots, so the memory-bound argmin
 // of early tasks overlaps the arithmetic-bound aggregation of later ones
-// instead of running after all of it.  Waves whose block index modulo 4 is
-// below `wta_quarters` look for argmin units first, the others for path
+// instead of running after all of it.  Waves whose block index is a multiple
+// of 4 (a quarter of them) look for argmin units first, the others for path
 // items first; both take the other kind when theirs has run out, and a wave
 // with neither sleeps briefly while path items are still in flight.  Every
 // claimed path item is held by a running wave, so every task is published
@@ -365,12 +365,12 @@ __global__ __launch_bounds__(256) void k_fused_reset(int *__restrict__ sync, con
 __global__ __launch_bounds__(64) void k_batch_fused(const BatchPathTask *__restrict__ tasks,
                                                     const uint32_t *__restrict__ items, int nitems,
                                                     const WtaTask *__restrict__ wtas, int nwta,
-                                                    int *__restrict__ sync, int wta_quarters)
+                                                    int *__restrict__ sync)
 {
     int *head = sync, *pub = sync + kCounterStride;
     int *left = sync + 2 * kCounterStride;
     int *ready = left + nwta * kCounterStride, *taken = ready + nwta * kCounterStride;
-    const bool wta_first = (int)(blockIdx.x & 3) < wta_quarters;
+    const bool wta_first = (blockIdx.x & 3) == 0;
     bool paths_open = true;
     int s = 0, acquired = -1; // this wave's slot cursor; the last slot it acquired
 
@@ -453,7 +453,6 @@ struct Schedule {
     int census_shape = 0; // census tile kCenShapeW[i] x kCenShapeH[i]
     int xcd_slices = 0;   // 0: one global path queue; 1: per-XCD slices
     int fused = 0;        // 1: path and argmin in one launch (k_batch_fused; order and slices unused)
-    int wta_quarters = 1; // fused: quarters of the waves that look for argmin units first
 };
 
 } // namespace
@@ -610,7 +609,7 @@ int enqueue_lane(sgmh_batch *b, BatchLane &ln, hipStream_t s, const Schedule &sc
         const int waves = std::min(n, b->cus * 4 * sc.path_waves_per_simd);
         hipLaunchKernelGGL(k_fused_reset, dim3(1), dim3(256), 0, s, ln.d_sync, ln.left0.dev, nwta);
         hipLaunchKernelGGL(k_batch_fused, dim3(waves), dim3(64), 0, s, ln.path.dev, ln.d_items + 2 * n, n,
-                           ln.wta.dev, nwta, ln.d_sync, sc.wta_quarters);
+                           ln.wta.dev, nwta, ln.d_sync);
         CK(hipGetLastError());
         return 0;
     }
@@ -737,11 +736,11 @@ int autotune(sgmh_batch *b)
             char buf[320];
             snprintf(buf, sizeof buf,
                      "%s{\"lanes\":\"%s\",\"order\":\"%s\",\"path_waves_per_simd\":%d,\"census_tile\":\"%dx%d\","
-                     "\"scanline_grouping\":\"%s\",\"fused\":%s,\"argmin_first_quarters\":%d,\"ms\":%.4f}",
+                     "\"scanline_grouping\":\"%s\",\"fused\":%s,\"ms\":%.4f}",
                      ntimed++ ? "," : "", b->sets[c.set].name.c_str(), order_name(c.sc),
                      c.sc.path_waves_per_simd, kCenShapeW[c.sc.census_shape], kCenShapeH[c.sc.census_shape],
                      c.sc.xcd_slices ? "per-XCD" : "global", c.sc.fused ? "true" : "false",
-                     c.sc.fused ? c.sc.wta_quarters : 0, t[1]);
+                     t[1]);
             js += buf;
             if (t[1] < best) {
                 best = t[1];
@@ -1243,12 +1242,12 @@ int sgmh_batch_describe(const sgmh_batch *b, char *buf, size_t len)
         "{\"pairs\":%zu,\"plain\":%d,\"multiscale\":%d,\"workspace_bytes\":%zu,\"path_items\":%zu,"
         "\"path_tasks\":%zu,\"wta_tasks\":%zu,\"graph\":%s,\"schedule\":{\"lanes\":\"%s\",\"lane_count\":%zu,"
         "\"pipelined\":%s,\"order\":\"%s\",\"path_waves_per_simd\":%d,\"census_tile\":\"%dx%d\",\"scanline_grouping\":\"%s\","
-        "\"fused\":%s,\"argmin_first_quarters\":%d},\"tuning\":%s}",
+        "\"fused\":%s},\"tuning\":%s}",
         b->pairs.size(), plain, ms, b->arena_bytes, items, tasks, wtas, b->graph_ok ? "true" : "false",
         ls.name.c_str(), ls.lanes.size(), ls.pipelined ? "true" : "false", order_name(b->sched),
         b->sched.path_waves_per_simd, kCenShapeW[b->sched.census_shape], kCenShapeH[b->sched.census_shape],
         b->sched.xcd_slices ? "per-XCD" : "global", b->sched.fused ? "true" : "false",
-        b->sched.fused ? b->sched.wta_quarters : 0, b->tuning.empty() ? "null" : b->tuning.c_str());
+        b->tuning.empty() ? "null" : b->tuning.c_str());
     return n;
 }
 
